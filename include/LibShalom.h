@@ -1,0 +1,149 @@
+/*
+ * LibShalom.h -- MI355X (gfx950) implementation of LibShalom's small and
+ * irregular-shaped GEMM API.
+ *
+ * Drop-in for the reference header NN_LIB/LibShalom.h with these changes:
+ *   - every symbol is extern "C" (the reference's are C++-mangled because
+ *     its header is C++, NN_LIB/LibShalom.h:5-7);
+ *   - A, B and C are DEVICE pointers (hipMalloc / torch tensors / managed
+ *     memory) on the current HIP device;
+ *   - transa is honoured (the reference ignores it, NN_SMM.c:8,
+ *     SMM_thread.c:81) and dgemm supports Trans (a no-op in the reference,
+ *     NN_SMM.c:33-37);
+ *   - only the public entry points are exported; the reference's internal
+ *     kernels and pack routines (SGEMM_NN, SGEMM_NT, Small_*, NPACK, TPACK,
+ *     Dete_grad_*) and the GEMM_K/N/M blocking macros have no counterpart
+ *     because the packing and partition are done inside the GPU kernels.
+ *
+ * Storage is row-major with the reference's implicit leading dimensions:
+ * op(A) is M x K (A stored M x K, lda = K; or Trans: K x M, lda = M),
+ * op(B) is K x N (B stored K x N, ldb = N; or Trans: N x K, ldb = K),
+ * C is M x N (ldc = N).  C = op(A) * op(B); C is overwritten (beta = 0).
+ */
+#ifndef LIBSHALOM_AMD_H
+#define LIBSHALOM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* NN_LIB/LibShalom.h:10-12 */
+#define NoTrans 0
+#define Trans 1
+
+/* A hipStream_t.  NULL is the null (default) stream. */
+typedef void *LibShalom_stream_t;
+
+/* Status codes (the reference has no error reporting; its functions return
+ * void and silently skip unsupported cases). */
+#define LIBSHALOM_SUCCESS 0
+#define LIBSHALOM_INVALID_VALUE 1   /* negative size, bad trans / precision flag */
+#define LIBSHALOM_INVALID_POINTER 2 /* NULL or non-device pointer for a non-empty operand */
+#define LIBSHALOM_HIP_ERROR 3       /* a HIP runtime call failed */
+#define LIBSHALOM_ALLOC_FAILED 4    /* workspace allocation failed */
+#define LIBSHALOM_NOT_SUPPORTED 5   /* size beyond the supported range (dims < 2^31) */
+#define LIBSHALOM_NO_DEVICE 6       /* no usable gfx950 device */
+
+/* Arithmetic of the float GEMMs.  FP32: FP32 MFMA.  BF16X3: each FP32
+ * operand split as hi + lo (bf16), three BF16 MFMA products hi*hi + hi*lo +
+ * lo*hi, FP32 accumulation.  BF16: one BF16 product.  BF16X6: three-way
+ * split, six products (about FP32 accuracy).  FP64 GEMMs always use FP64 MFMA. */
+#define LIBSHALOM_FP32 0
+#define LIBSHALOM_BF16X3 1
+#define LIBSHALOM_BF16 2
+#define LIBSHALOM_BF16X6 3
+#define LIBSHALOM_FP64 4 /* id of the DGEMM path in LibShalom_query_plan */
+
+/* ------------------------------------------------------------------------
+ * Reference API (synchronous: C is complete when the call returns).  Errors
+ * leave C untouched and are reported by LibShalom_get_last_error().
+ * ------------------------------------------------------------------------ */
+
+/* Replaces LibShalom_sgemm (NN_LIB/LibShalom.h:22, NN_SMM.c:5-27). */
+void LibShalom_sgemm(int transa, int transb, float *C, float *A, float *B, long M, long N, long K);
+
+/* Replaces LibShalom_sgemm_mp (NN_LIB/LibShalom.h:27, SMM_thread.c:75-114):
+ * the irregular-shape path (tall-skinny / short-wide); the OpenMP Tm x Tn
+ * thread grid becomes a workgroup schedule with split-K. */
+void LibShalom_sgemm_mp(int transa, int transb, float *C, float *A, float *B, long M, long N, long K);
+
+/* Replaces LibShalom_dgemm (NN_LIB/LibShalom.h:39, NN_SMM.c:30-38). */
+void LibShalom_dgemm(int transa, int transb, double *C, double *A, double *B, long M, long N, long K);
+
+/* Replace LibShalom_set_thread_nums / _set_thread_ways (NN_LIB/LibShalom.h:41-42,
+ * SMM_thread.c:116-137).  No-ops: the GPU schedule does not use a CPU thread
+ * count.  The values are recorded and can be read back. */
+void LibShalom_set_thread_nums(int num);
+void LibShalom_set_thread_ways(int tm, int tn);
+void LibShalom_get_thread_config(int *num, int *tm, int *tn);
+
+/* Precision used by the reference-API float calls above (default FP32).
+ * Returns LIBSHALOM_INVALID_VALUE for an unknown precision. */
+int LibShalom_set_precision(int precision);
+int LibShalom_get_precision(void);
+
+/* Status of the last LibShalom call made by this host thread. */
+int LibShalom_get_last_error(void);
+const char *LibShalom_status_string(int status);
+
+/* ------------------------------------------------------------------------
+ * Extended API (no reference counterpart): explicit precision and stream,
+ * asynchronous on `stream`, returns a status.
+ * ------------------------------------------------------------------------ */
+int LibShalom_sgemm_ex(int transa, int transb, float *C, const float *A, const float *B,
+                       long M, long N, long K, int precision, LibShalom_stream_t stream);
+int LibShalom_sgemm_mp_ex(int transa, int transb, float *C, const float *A, const float *B,
+                          long M, long N, long K, int precision, LibShalom_stream_t stream);
+int LibShalom_dgemm_ex(int transa, int transb, double *C, const double *A, const double *B,
+                       long M, long N, long K, LibShalom_stream_t stream);
+
+/* Batched GEMMs of uneven sizes (no reference counterpart; the reference
+ * makes one call per GEMM, e.g. benchmark/CP2K_kernels/LibShalom_dgemm.c:102-106).
+ * A, B, C, M, N, K are HOST arrays of `count` entries (device pointers and
+ * sizes); one transa / transb for the whole batch.
+ *
+ * A plan buckets the GEMMs by tile shape, uploads the work descriptors once
+ * and captures the launches in a hipGraph; LibShalom_plan_execute replays it
+ * on `stream`.  The pointers are baked into the plan. */
+typedef struct LibShalom_plan_s *LibShalom_plan_t;
+int LibShalom_sgemm_batched_plan(LibShalom_plan_t *plan, int transa, int transb, float *const *C,
+                                 const float *const *A, const float *const *B, const long *M,
+                                 const long *N, const long *K, long count, int precision);
+int LibShalom_dgemm_batched_plan(LibShalom_plan_t *plan, int transa, int transb, double *const *C,
+                                 const double *const *A, const double *const *B, const long *M,
+                                 const long *N, const long *K, long count);
+int LibShalom_plan_execute(LibShalom_plan_t plan, LibShalom_stream_t stream);
+int LibShalom_plan_destroy(LibShalom_plan_t plan);
+/* flops = sum 2*M*N*K; bytes = sum s*(M*K + K*N + M*N) (algorithmic). */
+int LibShalom_plan_info(LibShalom_plan_t plan, double *flops, double *bytes, long *work_items);
+
+/* One-shot batched calls: plan, execute on `stream`, wait, destroy. */
+int LibShalom_sgemm_batched(int transa, int transb, float *const *C, const float *const *A,
+                            const float *const *B, const long *M, const long *N, const long *K,
+                            long count, int precision, LibShalom_stream_t stream);
+int LibShalom_dgemm_batched(int transa, int transb, double *const *C, const double *const *A,
+                            const double *const *B, const long *M, const long *N, const long *K,
+                            long count, LibShalom_stream_t stream);
+
+/* Backward of C = op(A) op(B) given dC (M x N): writes dA with the storage
+ * shape of A and dB with the storage shape of B,
+ *   op(A)' = dC op(B)^T,   op(B)' = op(A)^T dC,
+ * both GEMMs in one launch (no reference counterpart). */
+int LibShalom_sgemm_backward(int transa, int transb, const float *dC, const float *A,
+                             const float *B, float *dA, float *dB, long M, long N, long K,
+                             int precision, LibShalom_stream_t stream);
+
+/* Planner introspection (host only, no device needed): the workgroup tile
+ * (tile_m x tile_n of C), K-split count and K slice length the library would
+ * use for one GEMM on a device with num_cu compute units (mp != 0: the
+ * LibShalom_sgemm_mp path), and the tile a batched plan gives one GEMM.
+ * Replaces the reference's thread-grid planner Dete_grad_N_threads_nums /
+ * Dete_grad_M_threads_nums (SMM_thread.c:14-72). */
+int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_cu, int *tile_m,
+                         int *tile_n, int *ksplit, int *kper);
+int LibShalom_query_batched_tile(int precision, long M, long N, long K, int *tile_m, int *tile_n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LIBSHALOM_AMD_H */

@@ -1,0 +1,608 @@
+// api.cpp -- the extern "C" boundary of libshalom.so: argument checking, the
+// tile / split-K planner that replaces the reference's OpenMP thread-grid
+// planner, per-stream workspaces, and batched plans captured in hipGraphs.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/LibShalom.h"
+#include "kernels.hpp"
+#include "planner.hpp"
+
+using namespace ls;
+
+namespace ls {
+LaunchTable &launch_table()
+{
+    static LaunchTable *t = [] {
+        LaunchTable *p = new LaunchTable();
+        std::memset(p, 0, sizeof(LaunchTable));
+        register_fp32(*p);
+        register_bf16x3(*p);
+        register_bf16(*p);
+        register_bf16x6(*p);
+        register_fp64(*p);
+        return p;
+    }();
+    return *t;
+}
+}  // namespace ls
+
+namespace {
+
+thread_local int g_last_status = LIBSHALOM_SUCCESS;
+std::atomic<int> g_precision{LIBSHALOM_FP32};
+std::atomic<int> g_thread_nums{64}, g_tm{0}, g_tn{0};
+
+int set_status(int s)
+{
+    g_last_status = s;
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// Devices.
+// ---------------------------------------------------------------------------
+struct DeviceInfo {
+    bool checked = false;
+    bool ok = false;
+    int num_cu = 256;
+};
+std::mutex g_dev_mu;
+std::map<int, DeviceInfo> g_devices;
+
+int current_device(int *dev, DeviceInfo *info)
+{
+    int d = 0;
+    if (hipGetDevice(&d) != hipSuccess) {
+        (void)hipGetLastError();
+        return LIBSHALOM_NO_DEVICE;
+    }
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    DeviceInfo &di = g_devices[d];
+    if (!di.checked) {
+        di.checked = true;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, d) == hipSuccess) {
+            di.ok = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
+            di.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    if (!di.ok) return LIBSHALOM_NO_DEVICE;
+    *dev = d;
+    *info = di;
+    return LIBSHALOM_SUCCESS;
+}
+
+// A pointer the GPU can dereference: device, managed or pinned-host memory.
+bool device_accessible(const void *p)
+{
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged ||
+           attr.type == hipMemoryTypeUnified || attr.devicePointer != nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Per (device, stream) workspace for split-K partial sums.
+// ---------------------------------------------------------------------------
+struct Workspace {
+    void *ptr = nullptr;
+    size_t size = 0;
+};
+std::mutex g_ws_mu;
+std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+
+int workspace(int dev, hipStream_t s, size_t bytes, void **out)
+{
+    if (const char *f = std::getenv("LIBSHALOM_FAIL_WORKSPACE"))
+        if (f[0] == '1') return LIBSHALOM_ALLOC_FAILED;  // failure injection for tests
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    Workspace &w = g_ws[{dev, s}];
+    if (w.size < bytes) {
+        if (w.ptr) {
+            // earlier kernels on this stream may still read the old buffer
+            if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+            (void)hipFree(w.ptr);
+            w.ptr = nullptr;
+            w.size = 0;
+        }
+        size_t sz = std::max(bytes, (size_t)1 << 20);
+        if (hipMalloc(&w.ptr, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            w.ptr = nullptr;
+            return LIBSHALOM_ALLOC_FAILED;
+        }
+        w.size = sz;
+    }
+    *out = w.ptr;
+    return LIBSHALOM_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// Argument checks shared by every entry point.
+// ---------------------------------------------------------------------------
+int check_dims(int ta, int tb, long M, long N, long K)
+{
+    if ((ta != 0 && ta != 1) || (tb != 0 && tb != 1)) return LIBSHALOM_INVALID_VALUE;
+    if (M < 0 || N < 0 || K < 0) return LIBSHALOM_INVALID_VALUE;
+    const long lim = (1L << 31) - 1;
+    if (M > lim || N > lim || K > lim) return LIBSHALOM_NOT_SUPPORTED;
+    if ((double)M * (double)K > 9.0e18 || (double)K * (double)N > 9.0e18 || (double)M * (double)N > 9.0e18)
+        return LIBSHALOM_NOT_SUPPORTED;
+    return LIBSHALOM_SUCCESS;
+}
+
+int check_ptrs(const void *C, const void *A, const void *B, long M, long N, long K, bool deep)
+{
+    if (M == 0 || N == 0) return LIBSHALOM_SUCCESS;
+    if (!C) return LIBSHALOM_INVALID_POINTER;
+    if (K > 0 && (!A || !B)) return LIBSHALOM_INVALID_POINTER;
+    if (deep) {
+        if (!device_accessible(C)) return LIBSHALOM_INVALID_POINTER;
+        if (K > 0 && (!device_accessible(A) || !device_accessible(B))) return LIBSHALOM_INVALID_POINTER;
+    }
+    return LIBSHALOM_SUCCESS;
+}
+
+GemmDesc make_desc(int ta, int tb, const void *A, const void *B, void *C, long M, long N, long K, int esize)
+{
+    GemmDesc d;
+    d.A = A;
+    d.B = B;
+    d.C = C;
+    d.M = (int)M;
+    d.N = (int)N;
+    d.K = (int)K;
+    d.lda = (int)(ta ? M : K);
+    d.ldb = (int)(tb ? K : N);
+    d.ldc = (int)N;
+    const int vec = 16 / esize;
+    int flags = 0;
+    if (((uintptr_t)A % 16) == 0 && d.lda % vec == 0) flags |= 1;
+    if (((uintptr_t)B % 16) == 0 && d.ldb % vec == 0) flags |= 2;
+    d.flags = flags;
+    return d;
+}
+
+// One GEMM on `stream`: plan tiles / split-K and launch.
+int run_single(int prec, int ta, int tb, void *C, const void *A, const void *B, long M, long N, long K,
+               bool mp, hipStream_t stream)
+{
+    int dev;
+    DeviceInfo info;
+    int st = current_device(&dev, &info);
+    if (st) return st;
+    if (M == 0 || N == 0) return LIBSHALOM_SUCCESS;
+    const bool f64 = prec == P_FP64;
+    const int esize = f64 ? 8 : 4;
+    GemmDesc d = make_desc(ta, tb, A, B, C, M, N, K, esize);
+    SinglePlan p = plan_single(prec, M, N, K, mp, info.num_cu);
+    TilesLaunch fn = launch_table().tiles[prec][ta][tb][p.tile];
+    if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    void *partial = nullptr;
+    if (p.ksplit > 1) {
+        st = workspace(dev, stream, (size_t)p.ksplit * M * N * esize, &partial);
+        if (st) return st;
+    }
+    if (fn(d, p.ksplit, p.kper, partial, stream) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    if (p.ksplit > 1) {
+        const long mn = M * N;
+        const int blocks = (int)std::min<long>((mn + 255) / 256, 4L * info.num_cu);
+        if (f64)
+            hipLaunchKernelGGL(k_splitk_reduce<double>, dim3(blocks), dim3(256), 0, stream,
+                               (const double *)partial, (double *)C, mn, p.ksplit);
+        else
+            hipLaunchKernelGGL(k_splitk_reduce<float>, dim3(blocks), dim3(256), 0, stream,
+                               (const float *)partial, (float *)C, mn, p.ksplit);
+        if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    }
+    return LIBSHALOM_SUCCESS;
+}
+
+int sync_null()
+{
+    if (hipStreamSynchronize(nullptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return LIBSHALOM_HIP_ERROR;
+    }
+    return LIBSHALOM_SUCCESS;
+}
+
+int valid_precision(int p)
+{
+    return p == LIBSHALOM_FP32 || p == LIBSHALOM_BF16X3 || p == LIBSHALOM_BF16 || p == LIBSHALOM_BF16X6;
+}
+
+// LIBSHALOM_* precision flag -> kernel precision id (they coincide).
+int kprec(int p) { return p; }
+
+int sgemm_sync(int ta, int tb, float *C, float *A, float *B, long M, long N, long K, bool mp)
+{
+    int st = check_dims(ta, tb, M, N, K);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, true);
+    if (!st) st = run_single(kprec(g_precision.load()), ta, tb, C, A, B, M, N, K, mp, nullptr);
+    if (!st) st = sync_null();
+    return set_status(st);
+}
+
+}  // namespace
+
+// ===========================================================================
+// Batched plans.
+// ===========================================================================
+struct LibShalom_plan_s {
+    int device = 0;
+    int prec = 0;
+    void *dev_buf = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double flops = 0, bytes = 0;
+    long items = 0;
+};
+
+namespace {
+
+struct Bucket {
+    int tile;
+    std::vector<GemmDesc> descs;
+    std::vector<WorkItem> items;
+};
+
+int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, const void *const *A,
+               const void *const *B, const long *M, const long *N, const long *K, long count)
+{
+    if (!out) return LIBSHALOM_INVALID_POINTER;
+    *out = nullptr;
+    if (count < 0) return LIBSHALOM_INVALID_VALUE;
+    if (count > 0 && (!C || !A || !B || !M || !N || !K)) return LIBSHALOM_INVALID_POINTER;
+    if ((ta != 0 && ta != 1) || (tb != 0 && tb != 1)) return LIBSHALOM_INVALID_VALUE;
+    int dev;
+    DeviceInfo info;
+    int st = current_device(&dev, &info);
+    if (st) return st;
+    const bool f64 = prec == P_FP64;
+    const int esize = f64 ? 8 : 4;
+
+    std::vector<Bucket> buckets(NTILE);
+    for (int t = 0; t < NTILE; ++t) buckets[t].tile = t;
+    double flops = 0, bytes = 0;
+    for (long g = 0; g < count; ++g) {
+        st = check_dims(ta, tb, M[g], N[g], K[g]);
+        if (st) return st;
+        st = check_ptrs(C[g], A[g], B[g], M[g], N[g], K[g], false);
+        if (st) return st;
+        if (M[g] == 0 || N[g] == 0) continue;
+        flops += 2.0 * M[g] * N[g] * K[g];
+        bytes += (double)esize * (M[g] * K[g] + K[g] * N[g] + M[g] * N[g]);
+        const int tile = batched_tile(prec, M[g], N[g], K[g]);
+        buckets[tile].descs.push_back(make_desc(ta, tb, A[g], B[g], C[g], M[g], N[g], K[g], esize));
+    }
+    // Longest GEMMs first inside a bucket; biggest tiles first across buckets.
+    long total_items = 0;
+    for (auto &b : buckets) {
+        std::stable_sort(b.descs.begin(), b.descs.end(), [](const GemmDesc &x, const GemmDesc &y) {
+            return (double)x.M * x.N * x.K > (double)y.M * y.N * y.K;
+        });
+        const TileShape ts = kTileShapes[b.tile];
+        for (int i = 0; i < (int)b.descs.size(); ++i) {
+            const GemmDesc &d = b.descs[i];
+            const int nt = ((d.M + ts.bm - 1) / ts.bm) * ((d.N + ts.bn - 1) / ts.bn);
+            for (int t = 0; t < nt; ++t) b.items.push_back(WorkItem{i, t});
+        }
+        total_items += (long)b.items.size();
+        if ((long)b.items.size() > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
+    }
+
+    LibShalom_plan_s *plan = new (std::nothrow) LibShalom_plan_s();
+    if (!plan) return LIBSHALOM_ALLOC_FAILED;
+    plan->device = dev;
+    plan->prec = prec;
+    plan->flops = flops;
+    plan->bytes = bytes;
+    plan->items = total_items;
+
+    size_t total = 0;
+    std::vector<std::pair<size_t, size_t>> offs(NTILE);
+    for (auto &b : buckets) {
+        offs[b.tile].first = total;
+        total += (b.descs.size() * sizeof(GemmDesc) + 255) & ~(size_t)255;
+        offs[b.tile].second = total;
+        total += (b.items.size() * sizeof(WorkItem) + 255) & ~(size_t)255;
+    }
+    hipStream_t cs = nullptr;
+    auto fail = [&](int code) {
+        if (cs) (void)hipStreamDestroy(cs);
+        LibShalom_plan_destroy(plan);
+        (void)hipGetLastError();
+        return code;
+    };
+    if (total > 0) {
+        if (hipMalloc(&plan->dev_buf, total) != hipSuccess) {
+            plan->dev_buf = nullptr;
+            return fail(LIBSHALOM_ALLOC_FAILED);
+        }
+        std::vector<char> host(total);
+        for (auto &b : buckets) {
+            if (!b.descs.empty())
+                std::memcpy(host.data() + offs[b.tile].first, b.descs.data(), b.descs.size() * sizeof(GemmDesc));
+            if (!b.items.empty())
+                std::memcpy(host.data() + offs[b.tile].second, b.items.data(), b.items.size() * sizeof(WorkItem));
+        }
+        if (hipMemcpy(plan->dev_buf, host.data(), total, hipMemcpyHostToDevice) != hipSuccess)
+            return fail(LIBSHALOM_HIP_ERROR);
+    }
+    // Capture the bucket launches into one graph.
+    if (hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) != hipSuccess) return fail(LIBSHALOM_HIP_ERROR);
+    if (hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal) != hipSuccess) return fail(LIBSHALOM_HIP_ERROR);
+    const int order[NTILE] = {T128x128, T128x64, T128x32, T32x128, T64x64, T32x32};
+    bool launch_ok = true;
+    for (int oi = 0; oi < NTILE; ++oi) {
+        const Bucket &b = buckets[order[oi]];
+        if (b.items.empty()) continue;
+        BatchedLaunch fn = launch_table().batched[prec][ta][tb][b.tile];
+        const GemmDesc *dd = (const GemmDesc *)((char *)plan->dev_buf + offs[b.tile].first);
+        const WorkItem *di = (const WorkItem *)((char *)plan->dev_buf + offs[b.tile].second);
+        if (!fn || fn(dd, di, (int)b.items.size(), cs) != hipSuccess) launch_ok = false;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(cs, &g);
+    plan->graph = g;
+    if (!launch_ok || e != hipSuccess) return fail(LIBSHALOM_HIP_ERROR);
+    if (hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0) != hipSuccess) {
+        plan->exec = nullptr;
+        return fail(LIBSHALOM_HIP_ERROR);
+    }
+    (void)hipStreamDestroy(cs);
+    *out = plan;
+    return LIBSHALOM_SUCCESS;
+}
+
+}  // namespace
+
+// ===========================================================================
+// extern "C" entry points.
+// ===========================================================================
+extern "C" {
+
+void LibShalom_sgemm(int transa, int transb, float *C, float *A, float *B, long M, long N, long K)
+{
+    sgemm_sync(transa, transb, C, A, B, M, N, K, false);
+}
+
+void LibShalom_sgemm_mp(int transa, int transb, float *C, float *A, float *B, long M, long N, long K)
+{
+    sgemm_sync(transa, transb, C, A, B, M, N, K, true);
+}
+
+void LibShalom_dgemm(int transa, int transb, double *C, double *A, double *B, long M, long N, long K)
+{
+    int st = check_dims(transa, transb, M, N, K);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, true);
+    if (!st) st = run_single(P_FP64, transa, transb, C, A, B, M, N, K, false, nullptr);
+    if (!st) st = sync_null();
+    set_status(st);
+}
+
+void LibShalom_set_thread_nums(int num)
+{
+    g_thread_nums.store(num);
+    set_status(LIBSHALOM_SUCCESS);
+}
+
+void LibShalom_set_thread_ways(int tm, int tn)
+{
+    g_tm.store(tm);
+    g_tn.store(tn);
+    g_thread_nums.store(tm * tn);
+    set_status(LIBSHALOM_SUCCESS);
+}
+
+void LibShalom_get_thread_config(int *num, int *tm, int *tn)
+{
+    if (num) *num = g_thread_nums.load();
+    if (tm) *tm = g_tm.load();
+    if (tn) *tn = g_tn.load();
+}
+
+int LibShalom_set_precision(int precision)
+{
+    if (!valid_precision(precision)) return set_status(LIBSHALOM_INVALID_VALUE);
+    g_precision.store(precision);
+    return set_status(LIBSHALOM_SUCCESS);
+}
+
+int LibShalom_get_precision(void) { return g_precision.load(); }
+
+int LibShalom_get_last_error(void) { return g_last_status; }
+
+const char *LibShalom_status_string(int status)
+{
+    switch (status) {
+    case LIBSHALOM_SUCCESS: return "success";
+    case LIBSHALOM_INVALID_VALUE: return "invalid value";
+    case LIBSHALOM_INVALID_POINTER: return "invalid pointer";
+    case LIBSHALOM_HIP_ERROR: return "HIP runtime error";
+    case LIBSHALOM_ALLOC_FAILED: return "workspace allocation failed";
+    case LIBSHALOM_NOT_SUPPORTED: return "not supported";
+    case LIBSHALOM_NO_DEVICE: return "no gfx950 device";
+    default: return "unknown status";
+    }
+}
+
+int LibShalom_sgemm_ex(int transa, int transb, float *C, const float *A, const float *B, long M, long N, long K,
+                       int precision, LibShalom_stream_t stream)
+{
+    int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
+    if (!st) st = check_dims(transa, transb, M, N, K);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    if (!st) st = run_single(kprec(precision), transa, transb, C, A, B, M, N, K, false, (hipStream_t)stream);
+    return set_status(st);
+}
+
+int LibShalom_sgemm_mp_ex(int transa, int transb, float *C, const float *A, const float *B, long M, long N,
+                          long K, int precision, LibShalom_stream_t stream)
+{
+    int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
+    if (!st) st = check_dims(transa, transb, M, N, K);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    if (!st) st = run_single(kprec(precision), transa, transb, C, A, B, M, N, K, true, (hipStream_t)stream);
+    return set_status(st);
+}
+
+int LibShalom_dgemm_ex(int transa, int transb, double *C, const double *A, const double *B, long M, long N, long K,
+                       LibShalom_stream_t stream)
+{
+    int st = check_dims(transa, transb, M, N, K);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    if (!st) st = run_single(P_FP64, transa, transb, C, A, B, M, N, K, false, (hipStream_t)stream);
+    return set_status(st);
+}
+
+int LibShalom_sgemm_batched_plan(LibShalom_plan_t *plan, int transa, int transb, float *const *C,
+                                 const float *const *A, const float *const *B, const long *M, const long *N,
+                                 const long *K, long count, int precision)
+{
+    if (!valid_precision(precision)) return set_status(LIBSHALOM_INVALID_VALUE);
+    return set_status(build_plan(plan, kprec(precision), transa, transb, (void *const *)C, (const void *const *)A,
+                                 (const void *const *)B, M, N, K, count));
+}
+
+int LibShalom_dgemm_batched_plan(LibShalom_plan_t *plan, int transa, int transb, double *const *C,
+                                 const double *const *A, const double *const *B, const long *M, const long *N,
+                                 const long *K, long count)
+{
+    return set_status(build_plan(plan, P_FP64, transa, transb, (void *const *)C, (const void *const *)A,
+                                 (const void *const *)B, M, N, K, count));
+}
+
+int LibShalom_plan_execute(LibShalom_plan_t plan, LibShalom_stream_t stream)
+{
+    if (!plan || !plan->exec) return set_status(LIBSHALOM_INVALID_POINTER);
+    if (hipGraphLaunch(plan->exec, (hipStream_t)stream) != hipSuccess) {
+        (void)hipGetLastError();
+        return set_status(LIBSHALOM_HIP_ERROR);
+    }
+    return set_status(LIBSHALOM_SUCCESS);
+}
+
+int LibShalom_plan_destroy(LibShalom_plan_t plan)
+{
+    if (!plan) return set_status(LIBSHALOM_SUCCESS);
+    if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+    if (plan->graph) (void)hipGraphDestroy(plan->graph);
+    if (plan->dev_buf) (void)hipFree(plan->dev_buf);
+    delete plan;
+    return set_status(LIBSHALOM_SUCCESS);
+}
+
+int LibShalom_plan_info(LibShalom_plan_t plan, double *flops, double *bytes, long *work_items)
+{
+    if (!plan) return set_status(LIBSHALOM_INVALID_POINTER);
+    if (flops) *flops = plan->flops;
+    if (bytes) *bytes = plan->bytes;
+    if (work_items) *work_items = plan->items;
+    return set_status(LIBSHALOM_SUCCESS);
+}
+
+static int batched_oneshot(int prec, int ta, int tb, void *const *C, const void *const *A, const void *const *B,
+                           const long *M, const long *N, const long *K, long count, hipStream_t s)
+{
+    LibShalom_plan_t plan = nullptr;
+    int st = build_plan(&plan, prec, ta, tb, C, A, B, M, N, K, count);
+    if (st) return st;
+    if (hipGraphLaunch(plan->exec, s) != hipSuccess) st = LIBSHALOM_HIP_ERROR;
+    if (hipStreamSynchronize(s) != hipSuccess) st = LIBSHALOM_HIP_ERROR;
+    (void)hipGetLastError();
+    LibShalom_plan_destroy(plan);
+    return st;
+}
+
+int LibShalom_sgemm_batched(int transa, int transb, float *const *C, const float *const *A, const float *const *B,
+                            const long *M, const long *N, const long *K, long count, int precision,
+                            LibShalom_stream_t stream)
+{
+    if (!valid_precision(precision)) return set_status(LIBSHALOM_INVALID_VALUE);
+    return set_status(batched_oneshot(kprec(precision), transa, transb, (void *const *)C, (const void *const *)A,
+                                      (const void *const *)B, M, N, K, count, (hipStream_t)stream));
+}
+
+int LibShalom_dgemm_batched(int transa, int transb, double *const *C, const double *const *A,
+                            const double *const *B, const long *M, const long *N, const long *K, long count,
+                            LibShalom_stream_t stream)
+{
+    return set_status(batched_oneshot(P_FP64, transa, transb, (void *const *)C, (const void *const *)A,
+                                      (const void *const *)B, M, N, K, count, (hipStream_t)stream));
+}
+
+int LibShalom_sgemm_backward(int transa, int transb, const float *dC, const float *A, const float *B, float *dA,
+                             float *dB, long M, long N, long K, int precision, LibShalom_stream_t stream)
+{
+    int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
+    if (!st) st = check_dims(transa, transb, M, N, K);
+    if (st) return set_status(st);
+    if (M == 0 || N == 0 || K == 0) {
+        // dA / dB are M x K and K x N: zero-sized or all zeros.
+        if (K > 0 && M > 0 && dA && hipMemsetAsync(dA, 0, sizeof(float) * M * K, (hipStream_t)stream) != hipSuccess)
+            return set_status(LIBSHALOM_HIP_ERROR);
+        if (K > 0 && N > 0 && dB && hipMemsetAsync(dB, 0, sizeof(float) * K * N, (hipStream_t)stream) != hipSuccess)
+            return set_status(LIBSHALOM_HIP_ERROR);
+        return set_status(LIBSHALOM_SUCCESS);
+    }
+    if (!dC || !A || !B || !dA || !dB) return set_status(LIBSHALOM_INVALID_POINTER);
+    const int p = kprec(precision);
+    hipStream_t s = (hipStream_t)stream;
+    // op(A)' = dC op(B)^T  (M x K); stored transposed when transa.
+    if (!transa)
+        st = run_single(p, 0, 1 - transb, dA, dC, B, M, K, N, false, s);
+    else
+        st = run_single(p, transb, 1, dA, B, dC, K, M, N, false, s);
+    if (st) return set_status(st);
+    // op(B)' = op(A)^T dC  (K x N); stored transposed when transb.
+    if (!transb)
+        st = run_single(p, 1 - transa, 0, dB, A, dC, K, N, M, false, s);
+    else
+        st = run_single(p, 1, transa, dB, dC, A, N, K, M, false, s);
+    return set_status(st);
+}
+
+int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_cu, int *tile_m, int *tile_n,
+                         int *ksplit, int *kper)
+{
+    if (!valid_precision(precision) && precision != LIBSHALOM_FP64) return set_status(LIBSHALOM_INVALID_VALUE);
+    if (M < 0 || N < 0 || K < 0 || num_cu <= 0) return set_status(LIBSHALOM_INVALID_VALUE);
+    const SinglePlan p = plan_single(kprec(precision), M, N, K, mp != 0, num_cu);
+    if (tile_m) *tile_m = kTileShapes[p.tile].bm;
+    if (tile_n) *tile_n = kTileShapes[p.tile].bn;
+    if (ksplit) *ksplit = p.ksplit;
+    if (kper) *kper = p.kper;
+    return set_status(LIBSHALOM_SUCCESS);
+}
+
+int LibShalom_query_batched_tile(int precision, long M, long N, long K, int *tile_m, int *tile_n)
+{
+    if (!valid_precision(precision) && precision != LIBSHALOM_FP64) return set_status(LIBSHALOM_INVALID_VALUE);
+    if (M < 0 || N < 0 || K < 0) return set_status(LIBSHALOM_INVALID_VALUE);
+    const int t = batched_tile(kprec(precision), M, N, K);
+    if (tile_m) *tile_m = kTileShapes[t].bm;
+    if (tile_n) *tile_n = kTileShapes[t].bn;
+    return set_status(LIBSHALOM_SUCCESS);
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// kernels.hpp -- __global__ entry points built on the tile GEMM, and the
+// host-side launcher table each precision's translation unit fills in.
+#pragma once
+#include "gemm_core.hpp"
+
+namespace ls {
+
+// Tile shapes the planner can choose from (rows x cols of C per workgroup).
+enum TileId : int { T32x32 = 0, T64x64, T128x128, T128x32, T128x64, T32x128, NTILE };
+struct TileShape { int bm, bn; };
+constexpr TileShape kTileShapes[NTILE] = {{32, 32}, {64, 64}, {128, 128}, {128, 32}, {128, 64}, {32, 128}};
+constexpr int NPREC = 5;
+
+// One GEMM (or one K slice of it) over a 1-D grid of tiles.  With ksplit > 1,
+// blockIdx.y is the K slice and the tile goes to partial[slice] (M x N, fp32
+// for the float modes, fp64 for FP64); a reduction kernel sums the slices in
+// slice order, so results do not depend on scheduling.
+template <class Cfg>
+__global__ __launch_bounds__(kThreads) void k_gemm_tiles(GemmDesc d, int tiles_n, int ntiles, int kper,
+                                                        typename Cfg::T *partial)
+{
+    __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
+    const int b = xcd_remap(blockIdx.x, ntiles);
+    const int tm = b / tiles_n, tn = b - tm * tiles_n;
+    const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
+    const int kbeg = blockIdx.y * kper;
+    const int kend = min(d.K, kbeg + kper);
+    Tile<Cfg> t;
+    t.run(d, m0, n0, kbeg, kend, lds);
+    if (gridDim.y == 1)
+        t.store((typename Cfg::T *)d.C, d.ldc, d.M, d.N, m0, n0);
+    else
+        t.store(partial + (size_t)blockIdx.y * d.M * d.N, d.N, d.M, d.N, m0, n0);
+}
+
+// Many independent GEMMs: one workgroup per (GEMM, tile) work item.
+template <class Cfg>
+__global__ __launch_bounds__(kThreads) void k_gemm_batched(const GemmDesc *__restrict__ descs,
+                                                          const WorkItem *__restrict__ items, int nitems)
+{
+    __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
+    const int b = xcd_remap(blockIdx.x, nitems);
+    const WorkItem it = items[b];
+    const GemmDesc d = descs[it.gemm];
+    const int tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
+    const int tm = it.tile / tiles_n, tn = it.tile - tm * tiles_n;
+    const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
+    Tile<Cfg> t;
+    t.run(d, m0, n0, 0, d.K, lds);
+    t.store((typename Cfg::T *)d.C, d.ldc, d.M, d.N, m0, n0);
+}
+
+// C = sum over slices of partial[s], in slice order.
+template <typename T>
+__global__ __launch_bounds__(256) void k_splitk_reduce(const T *__restrict__ partial, T *__restrict__ C,
+                                                       long mn, int nslices)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < mn; i += (long)gridDim.x * blockDim.x) {
+        T s = partial[i];
+        for (int j = 1; j < nslices; ++j) s += partial[(size_t)j * mn + i];
+        C[i] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launcher table (host side).
+// ---------------------------------------------------------------------------
+typedef hipError_t (*TilesLaunch)(const GemmDesc &d, int ksplit, int kper, void *partial, hipStream_t s);
+typedef hipError_t (*BatchedLaunch)(const GemmDesc *descs, const WorkItem *items, int nitems, hipStream_t s);
+
+struct LaunchTable {
+    TilesLaunch tiles[NPREC][2][2][NTILE];
+    BatchedLaunch batched[NPREC][2][2][NTILE];
+};
+LaunchTable &launch_table();
+
+template <class Cfg>
+hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, hipStream_t s)
+{
+    const int tiles_m = (d.M + Cfg::BM - 1) / Cfg::BM, tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
+    const int ntiles = tiles_m * tiles_n;
+    dim3 grid(ntiles, ksplit);
+    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(kThreads), 0, s, d, tiles_n, ntiles, kper,
+                       (typename Cfg::T *)partial);
+    return hipGetLastError();
+}
+
+template <class Cfg>
+hipError_t launch_batched(const GemmDesc *descs, const WorkItem *items, int nitems, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gemm_batched<Cfg>, dim3(nitems), dim3(kThreads), 0, s, descs, items, nitems);
+    return hipGetLastError();
+}
+
+template <int PREC, int TA, int TB, int BM, int BN>
+void register_one(LaunchTable &t, int tile)
+{
+    typedef TileCfg<PREC, TA, TB, BM, BN> Cfg;
+    t.tiles[PREC][TA][TB][tile] = &launch_tiles<Cfg>;
+    t.batched[PREC][TA][TB][tile] = &launch_batched<Cfg>;
+}
+
+template <int PREC, int TA, int TB>
+void register_trans(LaunchTable &t)
+{
+    register_one<PREC, TA, TB, 32, 32>(t, T32x32);
+    register_one<PREC, TA, TB, 64, 64>(t, T64x64);
+    register_one<PREC, TA, TB, 128, 128>(t, T128x128);
+    register_one<PREC, TA, TB, 128, 32>(t, T128x32);
+    register_one<PREC, TA, TB, 128, 64>(t, T128x64);
+    register_one<PREC, TA, TB, 32, 128>(t, T32x128);
+}
+
+// Defined in one translation unit per precision.
+void register_fp32(LaunchTable &t);
+void register_bf16x3(LaunchTable &t);
+void register_bf16(LaunchTable &t);
+void register_bf16x6(LaunchTable &t);
+void register_fp64(LaunchTable &t);
+
+}  // namespace ls

@@ -1,0 +1,181 @@
+/*
+ * oracle.c -- CPU restatement used as the checker for LibShalom's GEMM path.
+ * TEST INFRASTRUCTURE ONLY (see oracle.h for what each function restates
+ * and how it is pinned).  Built with -ffp-contract=off so the reference sums
+ * do not depend on the compiler's FMA contraction choices.
+ */
+#include "oracle.h"
+#include <math.h>
+#include <string.h>
+
+#define LCG_A 0x5DEECE66DULL
+#define LCG_C 0xBULL
+#define LCG_MASK ((1ULL << 48) - 1)
+
+/* Unseeded drand48 state on the reference's platform (Linux / glibc): glibc
+ * zero-initialises the 48-bit X and only fills in a and c on first use, so
+ * the first value is 11 / 2^48.  (0x1234ABCD330E is the BSD libc default;
+ * LibShalom's harnesses never call srand48 and run on Linux --
+ * benchmark/small_SGEMM/LibShalom_sgemm.c:26-34, NN_LIB/run.sh.) */
+void oracle_seed_default(uint64_t *state) { *state = 0ULL; }
+
+/* X_{n+1} = (a X_n + c) mod 2^48; drand48 returns X_{n+1} / 2^48. */
+double oracle_drand48(uint64_t *state)
+{
+    uint64_t x = (*state * LCG_A + LCG_C) & LCG_MASK;
+    *state = x;
+    return ldexp((double)x, -48);
+}
+
+void oracle_drand48_fill(uint64_t *state, double *out, long n)
+{
+    for (long i = 0; i < n; ++i) out[i] = oracle_drand48(state);
+}
+
+/* `a[i*n+j] = 2.0 * (float)drand48() - 1.0;` with a float destination:
+ * the double is rounded to float first, the affine map is done in double,
+ * then rounded to float again (benchmark/small_SGEMM/LibShalom_sgemm.c:33). */
+static inline float recipe_f32(uint64_t *state)
+{
+    float f = (float)oracle_drand48(state);
+    return (float)(2.0 * (double)f - 1.0);
+}
+
+void oracle_random_matrix(uint64_t *state, long m, long n, float *a)
+{
+    for (long i = 0; i < m; ++i)
+        for (long j = 0; j < n; ++j) a[i * n + j] = recipe_f32(state);
+}
+
+/* FP64 inputs: the same recipe with a double destination (the CP2K DGEMM
+ * harness leaves its inputs uninitialised, CP2K_kernels/LibShalom_dgemm.c:97-99,
+ * so the seeded drand48 stream stands in for it). */
+void oracle_random_matrix_d(uint64_t *state, long m, long n, double *a)
+{
+    for (long i = 0; i < m; ++i)
+        for (long j = 0; j < n; ++j) a[i * n + j] = 2.0 * oracle_drand48(state) - 1.0;
+}
+
+/* Hello.c:20-29: A row-major M x K, then B filled as B[i*K + j] for i < K,
+ * j < N (identical to row-major K x N only when N == K). */
+void oracle_hello_fill(uint64_t *state, long M, long N, long K, float *A, float *B)
+{
+    for (long i = 0; i < M; ++i)
+        for (long j = 0; j < K; ++j) A[i * K + j] = recipe_f32(state);
+    for (long i = 0; i < K; ++i)
+        for (long j = 0; j < N; ++j) B[i * K + j] = recipe_f32(state);
+}
+
+/* op(A)[i][k] and op(B)[k][j] for row-major storage with the reference's
+ * implicit leading dimensions. */
+#define OPA(i, k) (transa ? A[(k) * M + (i)] : A[(i) * K + (k)])
+#define OPB(k, j) (transb ? B[(j) * K + (k)] : B[(k) * N + (j)])
+
+void oracle_sgemm_ref(int transa, int transb, const float *A, const float *B,
+                      double *Cref, double *absAB, long M, long N, long K)
+{
+    for (long i = 0; i < M; ++i)
+        for (long j = 0; j < N; ++j) {
+            long double s = 0.0L, sa = 0.0L;
+            for (long k = 0; k < K; ++k) {
+                long double a = OPA(i, k), b = OPB(k, j);
+                s += a * b;
+                sa += fabsl(a) * fabsl(b);
+            }
+            Cref[i * N + j] = (double)s;
+            absAB[i * N + j] = (double)sa;
+        }
+}
+
+void oracle_dgemm_ref(int transa, int transb, const double *A, const double *B,
+                      double *Cref, double *absAB, long M, long N, long K)
+{
+    for (long i = 0; i < M; ++i)
+        for (long j = 0; j < N; ++j) {
+            long double s = 0.0L, sa = 0.0L;
+            for (long k = 0; k < K; ++k) {
+                long double a = OPA(i, k), b = OPB(k, j);
+                s += a * b;
+                sa += fabsl(a) * fabsl(b);
+            }
+            Cref[i * N + j] = (double)s;
+            absAB[i * N + j] = (double)sa;
+        }
+}
+
+#undef OPA
+#undef OPB
+
+void oracle_sgemm_ref_flat(int transa, int transb, long count, const float *A, const float *B,
+                           const long *offA, const long *offB, const long *offC, const long *M,
+                           const long *N, const long *K, double *Cref, double *absAB, double *Kelem)
+{
+    for (long g = 0; g < count; ++g) {
+        oracle_sgemm_ref(transa, transb, A + offA[g], B + offB[g], Cref + offC[g], absAB + offC[g],
+                         M[g], N[g], K[g]);
+        for (long i = 0; i < M[g] * N[g]; ++i) Kelem[offC[g] + i] = (double)K[g];
+    }
+}
+
+void oracle_dgemm_ref_flat(int transa, int transb, long count, const double *A, const double *B,
+                           const long *offA, const long *offB, const long *offC, const long *M,
+                           const long *N, const long *K, double *Cref, double *absAB, double *Kelem)
+{
+    for (long g = 0; g < count; ++g) {
+        oracle_dgemm_ref(transa, transb, A + offA[g], B + offB[g], Cref + offC[g], absAB + offC[g],
+                         M[g], N[g], K[g]);
+        for (long i = 0; i < M[g] * N[g]; ++i) Kelem[offC[g] + i] = (double)K[g];
+    }
+}
+
+static long check_core2(const void *Cv, int is_f64, const double *Cref, const double *absAB,
+                        const double *Kelem, long n, long K, double u, double extra, double *stats)
+{
+    const float *Cf = (const float *)Cv;
+    const double *Cd = (const double *)Cv;
+    double max_ratio = 0.0, max_rel = 0.0, sum_rel = 0.0, max_abs = 0.0;
+    long bad = 0, nrel = 0;
+    for (long i = 0; i < n; ++i) {
+        double kk = Kelem ? Kelem[i] : (double)K;
+        if (kk < 1.0) kk = 1.0;
+        double c = is_f64 ? Cd[i] : (double)Cf[i];
+        double err = fabs(c - Cref[i]);
+        if (isnan(c) || isnan(err)) { ++bad; max_ratio = INFINITY; continue; }
+        if (err > max_abs) max_abs = err;
+        double bound = (2.0 * kk * u + extra) * absAB[i];
+        if (absAB[i] == 0.0) {
+            if (err != 0.0) { ++bad; max_ratio = INFINITY; }
+            continue;
+        }
+        double ratio = err / bound;
+        double rel = err / absAB[i];
+        if (ratio > 1.0) ++bad;
+        if (ratio > max_ratio) max_ratio = ratio;
+        if (rel > max_rel) max_rel = rel;
+        sum_rel += rel;
+        ++nrel;
+    }
+    stats[0] = max_ratio;
+    stats[1] = max_rel;
+    stats[2] = nrel ? sum_rel / (double)nrel : 0.0;
+    stats[3] = max_abs;
+    return bad;
+}
+
+long oracle_check_f32(const float *C, const double *Cref, const double *absAB,
+                      long n, long K, double u, double *stats)
+{
+    return check_core2(C, 0, Cref, absAB, 0, n, K, u, 0.0, stats);
+}
+
+long oracle_check_f64(const double *C, const double *Cref, const double *absAB,
+                      long n, long K, double u, double *stats)
+{
+    return check_core2(C, 1, Cref, absAB, 0, n, K, u, 0.0, stats);
+}
+
+long oracle_check_kv(const void *C, int is_f64, const double *Cref, const double *absAB,
+                     const double *Kelem, long n, double u, double extra, double *stats)
+{
+    return check_core2(C, is_f64, Cref, absAB, Kelem, n, 0, u, extra, stats);
+}

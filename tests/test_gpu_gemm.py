@@ -1,0 +1,301 @@
+"""GPU parity tests: every entry point through the C-ABI against the oracle.
+
+Bar: |C - C_ref| <= 2*K*u*(|A||B|)_ij elementwise (u = 2^-24 for FP32 output,
+2^-53 for FP64), C prefilled with NaN so every element must be written.
+The BF16-family modes get their documented extra per-product term (see
+PREC_EXTRA).  Inputs follow the reference harness recipe 2*(float)drand48()-1
+from seeded drand48 streams.
+"""
+import numpy as np
+import pytest
+import torch
+
+import libshalom_amd as ls
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+U32 = 2.0 ** -24
+U64 = 2.0 ** -53
+# Extra relative error per product |a||b| of each float mode (from the split:
+# bf16 keeps 8 significand bits, so a two-piece split leaves ~2^-16 and a
+# three-piece split ~2^-24 of each operand; see DESIGN.md "Precision modes").
+PREC_EXTRA = {ls.FP32: 0.0, ls.BF16X6: 4 * 2.0 ** -24, ls.BF16X3: 3 * 2.0 ** -16, ls.BF16: 2 * 2.0 ** -8}
+TRANS = [(0, 0), (0, 1), (1, 0), (1, 1)]
+
+
+def dev_tensor(x, dev):
+    return torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+
+
+def nan_c(M, N, dev, dtype=torch.float32):
+    return torch.full((M, N), float("nan"), dtype=dtype, device=dev)
+
+
+def check_f32(C, A, B, ta, tb, M, N, K, prec=ls.FP32):
+    Cref, absAB = oracle.sgemm_ref(ta, tb, A, B, M, N, K)
+    Ke = np.full(M * N, float(max(K, 1)))
+    bad, st = oracle.check_kv(C.reshape(-1), Cref.reshape(-1), absAB.reshape(-1), Ke, U32, PREC_EXTRA[prec])
+    return bad, st
+
+
+def test_hello_80_nt(dev):
+    """Hello.c: LibShalom_sgemm(NoTrans, Trans, C, A, B, 80, 80, 80)."""
+    M = N = K = 80
+    A, B = oracle.hello_inputs(M, N, K)
+    dC = nan_c(M, N, dev)
+    ls.sgemm(ls.NoTrans, ls.Trans, dC, dev_tensor(A, dev), dev_tensor(B, dev), M, N, K)
+    C = dC.cpu().numpy()
+    bad, st = check_f32(C, A, B, 0, 1, M, N, K)
+    assert bad == 0, st
+    assert np.isfinite(C).all()
+
+
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_small_sweep_8_to_128(dev, ta, tb):
+    """benchmark/small_SGEMM: M = N = K = 8..128 step 8 (all four trans combos)."""
+    r = oracle.Drand48()
+    for n in range(8, 129, 8):
+        A = r.matrix(*oracle.a_shape(ta, n, n))
+        B = r.matrix(*oracle.b_shape(tb, n, n))
+        dC = nan_c(n, n, dev)
+        ls.sgemm(ta, tb, dC, dev_tensor(A, dev), dev_tensor(B, dev), n, n, n)
+        bad, st = check_f32(dC.cpu().numpy(), A, B, ta, tb, n, n, n)
+        assert bad == 0, (n, st)
+
+
+def _flat_batch(shapes, ta, tb, seed, f64=False):
+    r = oracle.Drand48(seed)
+    asz = [m * k for m, n, k in shapes]
+    bsz = [k * n for m, n, k in shapes]
+    csz = [m * n for m, n, k in shapes]
+    offA = np.concatenate([[0], np.cumsum(asz)]).astype(np.int64)
+    offB = np.concatenate([[0], np.cumsum(bsz)]).astype(np.int64)
+    offC = np.concatenate([[0], np.cumsum(csz)]).astype(np.int64)
+    if f64:
+        A = r.matrix_d(1, int(offA[-1])).ravel()
+        B = r.matrix_d(1, int(offB[-1])).ravel()
+    else:
+        A = r.matrix(1, int(offA[-1])).ravel()
+        B = r.matrix(1, int(offB[-1])).ravel()
+    return A, B, offA, offB, offC
+
+
+def _run_batched(dev, shapes, ta, tb, prec, seed, f64=False, replays=1):
+    A, B, offA, offB, offC = _flat_batch(shapes, ta, tb, seed, f64)
+    dt = torch.float64 if f64 else torch.float32
+    dA = dev_tensor(A, dev)
+    dB = dev_tensor(B, dev)
+    dC = torch.full((int(offC[-1]) + 1,), float("nan"), dtype=dt, device=dev)
+    es = dA.element_size()
+    Cs = [dC.data_ptr() + es * int(o) for o in offC[:-1]]
+    As = [dA.data_ptr() + es * int(o) for o in offA[:-1]]
+    Bs = [dB.data_ptr() + es * int(o) for o in offB[:-1]]
+    Ms, Ns, Ks = zip(*shapes)
+    outs = []
+    with ls.BatchedPlan(ta, tb, Cs, As, Bs, Ms, Ns, Ks, precision=prec, fp64=f64) as plan:
+        for _ in range(replays):
+            dC.fill_(float("nan"))
+            plan.execute()
+            torch.cuda.synchronize()
+            outs.append(dC.cpu().numpy().copy())
+    Cref, S, Ke = oracle.gemm_ref_flat(ta, tb, A, B, offA[:-1], offB[:-1], offC[:-1], Ms, Ns, Ks, int(offC[-1]))
+    return outs, Cref, S, Ke, offC
+
+
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_tiny_shapes_brute_force(dev, ta, tb):
+    """Every (M, N, K) in [1..33]^3 (one batched plan per trans combo)."""
+    shapes = [(m, n, k) for m in range(1, 34) for n in range(1, 34) for k in range(1, 34)]
+    outs, Cref, S, Ke, offC = _run_batched(dev, shapes, ta, tb, ls.FP32, 1000 + 2 * ta + tb)
+    C = outs[0][:-1]
+    assert np.isnan(outs[0][-1]), "wrote past the last C"
+    bad, st = oracle.check_kv(C, Cref, S, Ke, U32)
+    assert bad == 0, st
+
+
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_tiny_shapes_single_calls(dev, ta, tb):
+    rng = np.random.default_rng(5 + 2 * ta + tb)
+    r = oracle.Drand48(77 + ta + 2 * tb)
+    for _ in range(60):
+        M, N, K = (int(x) for x in rng.integers(1, 40, size=3))
+        A = r.matrix(*oracle.a_shape(ta, M, K))
+        B = r.matrix(*oracle.b_shape(tb, N, K))
+        dC = nan_c(M, N, dev)
+        ls.sgemm(ta, tb, dC, dev_tensor(A, dev), dev_tensor(B, dev), M, N, K)
+        bad, st = check_f32(dC.cpu().numpy(), A, B, ta, tb, M, N, K)
+        assert bad == 0, ((M, N, K), st)
+
+
+def test_all_ones_known_answer(dev):
+    for (M, N, K) in [(80, 80, 80), (33, 17, 129), (1, 300, 5)]:
+        for prec in (ls.FP32, ls.BF16X3, ls.BF16, ls.BF16X6):
+            A = torch.ones(M * K, device=dev)
+            B = torch.ones(K * N, device=dev)
+            dC = nan_c(M, N, dev)
+            ls.sgemm_ex(0, 0, dC, A, B, M, N, K, precision=prec)
+            torch.cuda.synchronize()
+            assert (dC == K).all(), (M, N, K, prec)
+
+
+def test_k_zero_and_empty(dev):
+    A = torch.zeros(1, device=dev)
+    dC = nan_c(5, 7, dev)
+    ls.sgemm(0, 0, dC, A, A, 5, 7, 0)
+    assert (dC == 0).all()
+    ls.sgemm(0, 0, dC, A, A, 0, 7, 3)  # empty: no-op
+    dD = torch.full((4, 3), float("nan"), dtype=torch.float64, device=dev)
+    ls.dgemm(1, 1, dD, A, A, 4, 3, 0)
+    assert (dD == 0).all()
+
+
+def _sample_rows_check(dC, dA, dB, ta, tb, M, N, K, prec, rng, nrows=48):
+    """Oracle check of a sample of C's rows (gathered on the device)."""
+    rows = np.unique(np.concatenate([[0, M - 1], rng.integers(0, M, size=nrows)]))
+    ridx = torch.from_numpy(rows).to(dC.device)
+    opA = dA.view(*oracle.a_shape(ta, M, K))
+    Asub = (opA[:, ridx].T if ta else opA[ridx]).contiguous().cpu().numpy()
+    C = dC.view(M, N)[ridx].cpu().numpy()
+    return check_f32(C, Asub, dB.cpu().numpy(), 0, tb, len(rows), N, K, prec)
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+@pytest.mark.parametrize("tb", [0, 1])
+def test_mp_tall_skinny(dev, prec, tb):
+    """Irregular-shaped sgemm_mp: M in {4096, 16384, 65536} x N in {16, 32, 64}
+    x K in {64, 256, 1024}; NN and NT; rows sampled for the oracle, NaN
+    prefill checks every element is written."""
+    rng = np.random.default_rng(11 + tb)
+    for M in (4096, 16384, 65536):
+        for N in (16, 32, 64):
+            for K in (64, 256, 1024):
+                gen = torch.Generator(device=dev).manual_seed(M + N + K)
+                dA = torch.rand(M * K, device=dev, generator=gen) * 2 - 1
+                dB = torch.rand(K * N, device=dev, generator=gen) * 2 - 1
+                dC = nan_c(M, N, dev)
+                ls.sgemm_mp_ex(0, tb, dC, dA, dB, M, N, K, precision=prec)
+                torch.cuda.synchronize()
+                assert bool(torch.isfinite(dC).all()), (M, N, K)
+                bad, st = _sample_rows_check(dC, dA, dB, 0, tb, M, N, K, prec, rng)
+                assert bad == 0, ((M, N, K), st)
+
+
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_mp_short_wide_and_transposes(dev, ta, tb):
+    rng = np.random.default_rng(3)
+    for (M, N, K) in [(16, 4096, 256), (64, 65536, 64), (5, 10000, 33), (256, 2048, 500), (333, 77, 1500)]:
+        gen = torch.Generator(device=dev).manual_seed(M * 7 + N + K)
+        dA = torch.rand(M * K, device=dev, generator=gen) * 2 - 1
+        dB = torch.rand(K * N, device=dev, generator=gen) * 2 - 1
+        dC = nan_c(M, N, dev)
+        ls.sgemm_mp(ta, tb, dC, dA, dB, M, N, K)
+        assert bool(torch.isfinite(dC).all()), (M, N, K)
+        bad, st = _sample_rows_check(dC, dA, dB, ta, tb, M, N, K, ls.FP32, rng)
+        assert bad == 0, ((M, N, K), st)
+
+
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_dgemm(dev, ta, tb):
+    """CP2K block shapes (benchmark/CP2K_kernels:74-76) plus the M % 7 in
+    {1, 3, 4} rows the reference's 7x6 kernel never writes (M = 4, 22, 32)."""
+    r = oracle.Drand48(2024 + ta + 2 * tb)
+    shapes = [(5, 5, 5), (13, 5, 13), (13, 13, 13), (23, 23, 23), (26, 26, 13), (4, 9, 16), (22, 4, 32),
+              (32, 32, 32), (1, 1, 1), (100, 70, 300), (513, 129, 67)]
+    for (M, N, K) in shapes:
+        A = r.matrix_d(*oracle.a_shape(ta, M, K))
+        B = r.matrix_d(*oracle.b_shape(tb, N, K))
+        dC = nan_c(M, N, dev, torch.float64)
+        ls.dgemm(ta, tb, dC, dev_tensor(A, dev), dev_tensor(B, dev), M, N, K)
+        Cref, absAB = oracle.dgemm_ref(ta, tb, A, B, M, N, K)
+        bad, st = oracle.check(dC.cpu().numpy(), Cref, absAB, K, U64)
+        assert bad == 0, ((M, N, K), st)
+
+
+@pytest.mark.parametrize("prec", [ls.BF16X3, ls.BF16, ls.BF16X6])
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_precision_modes(dev, prec, ta, tb):
+    r = oracle.Drand48(31 + prec)
+    for (M, N, K) in [(80, 80, 80), (128, 128, 128), (37, 91, 200), (7, 5, 3), (200, 64, 1000)]:
+        A = r.matrix(*oracle.a_shape(ta, M, K))
+        B = r.matrix(*oracle.b_shape(tb, N, K))
+        dC = nan_c(M, N, dev)
+        ls.sgemm_ex(ta, tb, dC, dev_tensor(A, dev), dev_tensor(B, dev), M, N, K, precision=prec)
+        torch.cuda.synchronize()
+        bad, st = check_f32(dC.cpu().numpy(), A, B, ta, tb, M, N, K, prec)
+        assert bad == 0, ((M, N, K), st)
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 1)])
+def test_batched_uneven(dev, prec, ta, tb):
+    """Uneven batch, M, N, K ~ U{1..128}; replayed twice: identical results."""
+    rng = np.random.default_rng(17 + prec)
+    shapes = [tuple(int(x) for x in rng.integers(1, 129, size=3)) for _ in range(400)]
+    outs, Cref, S, Ke, offC = _run_batched(dev, shapes, ta, tb, prec, 555, replays=2)
+    assert np.array_equal(outs[0], outs[1], equal_nan=True)
+    bad, st = oracle.check_kv(outs[0][:-1], Cref, S, Ke, U32, PREC_EXTRA[prec])
+    assert bad == 0, st
+    assert np.isnan(outs[0][-1])
+
+
+def test_dgemm_batched_cp2k(dev):
+    """FP64 batch with M, N, K drawn from {4,5,9,13,16,22,23,26,32}."""
+    dims = np.array([4, 5, 9, 13, 16, 22, 23, 26, 32])
+    rng = np.random.default_rng(4)
+    shapes = [tuple(int(x) for x in rng.choice(dims, size=3)) for _ in range(2000)]
+    outs, Cref, S, Ke, offC = _run_batched(dev, shapes, 0, 0, ls.FP64, 99, f64=True)
+    bad, st = oracle.check_kv(outs[0][:-1], Cref, S, Ke, U64)
+    assert bad == 0, st
+
+
+def test_one_shot_batched_matches_plan(dev):
+    rng = np.random.default_rng(8)
+    shapes = [tuple(int(x) for x in rng.integers(1, 70, size=3)) for _ in range(50)]
+    A, B, offA, offB, offC = _flat_batch(shapes, 0, 0, 8)
+    dA, dB = dev_tensor(A, dev), dev_tensor(B, dev)
+    dC = torch.full((int(offC[-1]),), float("nan"), device=dev)
+    Cs = [dC.data_ptr() + 4 * int(o) for o in offC[:-1]]
+    As = [dA.data_ptr() + 4 * int(o) for o in offA[:-1]]
+    Bs = [dB.data_ptr() + 4 * int(o) for o in offB[:-1]]
+    ls.sgemm_batched(0, 0, Cs, As, Bs, *zip(*shapes))
+    Cref, S, Ke = oracle.gemm_ref_flat(0, 0, A, B, offA[:-1], offB[:-1], offC[:-1], *zip(*shapes), int(offC[-1]))
+    assert oracle.check_kv(dC.cpu().numpy(), Cref, S, Ke, U32)[0] == 0
+
+
+def test_determinism_repeated_calls(dev):
+    r = oracle.Drand48(1)
+    M, N, K = 4096, 32, 1024
+    A = dev_tensor(r.matrix(M, K), dev)
+    B = dev_tensor(r.matrix(K, N), dev)
+    C1 = nan_c(M, N, dev)
+    C2 = nan_c(M, N, dev)
+    ls.sgemm_mp(0, 0, C1, A, B, M, N, K)
+    ls.sgemm_mp(0, 0, C2, A, B, M, N, K)
+    assert torch.equal(C1, C2)
+
+
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_backward_matches_matmul(dev, ta, tb):
+    torch.manual_seed(3)
+    for (M, N, K) in [(64, 300, 72), (17, 33, 5), (128, 49, 256)]:
+        A = torch.randn(*oracle.a_shape(ta, M, K), dtype=torch.float64, device=dev).float()
+        B = torch.randn(*oracle.b_shape(tb, N, K), dtype=torch.float64, device=dev).float()
+        dC = torch.randn(M, N, device=dev)
+        dA = torch.full_like(A, float("nan"))
+        dB = torch.full_like(B, float("nan"))
+        ls.sgemm_backward(ta, tb, dC, A, B, dA, dB, M, N, K)
+        torch.cuda.synchronize()
+        A64, B64, dC64 = A.double(), B.double(), dC.double()
+        opA = A64.T if ta else A64
+        opB = B64.T if tb else B64
+        gA = dC64 @ opB.T
+        gB = opA.T @ dC64
+        refA = gA.T if ta else gA
+        refB = gB.T if tb else gB
+        bndA = (dC64.abs() @ opB.abs().T)
+        bndB = (opA.abs().T @ dC64.abs())
+        bndA = (bndA.T if ta else bndA) * 2 * N * U32
+        bndB = (bndB.T if tb else bndB) * 2 * M * U32
+        assert ((dA.double() - refA).abs() <= bndA).all()
+        assert ((dB.double() - refB).abs() <= bndB).all()

@@ -1,0 +1,144 @@
+"""Host-side tests of the C-ABI library (no GPU): it loads, exports exactly
+what include/LibShalom.h declares, rejects bad arguments before touching the
+device, and its planner obeys its invariants."""
+import ctypes as ct
+import re
+
+import pytest
+
+import libshalom_amd as ls
+
+
+def _declared():
+    with open(ls.HEADER_PATH) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(LibShalom_\w+)\s*\(", text)))
+
+
+def test_header_symbols_exported():
+    names = _declared()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(ls.lib(), n), f"{n} declared but not exported"
+        assert n in ls.SIGNATURES, f"{n} has no ctypes signature"
+    assert sorted(ls.SIGNATURES) == names
+
+
+def test_reference_api_signatures_match_reference_order():
+    # C first, then A, B, then M, N, K (NN_LIB/LibShalom.h:22-39)
+    for n in ("LibShalom_sgemm", "LibShalom_sgemm_mp", "LibShalom_dgemm"):
+        res, args = ls.SIGNATURES[n]
+        assert res is None and len(args) == 8
+
+
+def test_libsmm_alias():
+    import os
+    alias = os.path.join(os.path.dirname(ls.LIB_PATH), "libsmm.so")
+    assert os.path.exists(alias)
+    ct.CDLL(alias).LibShalom_sgemm  # noqa: B018
+
+
+@pytest.mark.parametrize("args,code", [
+    ((0, 0, 0x1000, 0x1000, 0x1000, -1, 4, 4), 1),
+    ((0, 0, 0x1000, 0x1000, 0x1000, 4, -4, 4), 1),
+    ((2, 0, 0x1000, 0x1000, 0x1000, 4, 4, 4), 1),
+    ((0, 7, 0x1000, 0x1000, 0x1000, 4, 4, 4), 1),
+    ((0, 0, None, 0x1000, 0x1000, 4, 4, 4), 2),
+    ((0, 0, 0x1000, None, 0x1000, 4, 4, 4), 2),
+])
+def test_argument_errors_before_device(args, code):
+    L = ls.lib()
+    L.LibShalom_sgemm(*args)
+    assert L.LibShalom_get_last_error() == code
+    L.LibShalom_dgemm(*args)
+    assert L.LibShalom_get_last_error() == code
+    with pytest.raises(ls.LibShalomError) as e:
+        ls.sgemm_ex(*args, precision=ls.FP32, stream=0)
+    assert e.value.status == code
+
+
+def test_bad_precision_rejected():
+    with pytest.raises(ls.LibShalomError):
+        ls.set_precision(9)
+    with pytest.raises(ls.LibShalomError):
+        ls.sgemm_ex(0, 0, 0x1000, 0x1000, 0x1000, 4, 4, 4, precision=4, stream=0)
+    ls.set_precision(ls.BF16X3)
+    assert ls.get_precision() == ls.BF16X3
+    ls.set_precision(ls.FP32)
+    assert ls.get_precision() == ls.FP32
+
+
+def test_thread_config_is_recorded_noop():
+    ls.set_thread_nums(64)
+    assert ls.get_thread_config()[0] == 64
+    ls.set_thread_ways(8, 4)
+    assert ls.get_thread_config() == (32, 8, 4)
+
+
+def test_status_strings():
+    for s in range(7):
+        assert ls.lib().LibShalom_status_string(s)
+
+
+def test_batched_plan_rejects_bad_input():
+    h = ct.c_void_p()
+    L = ls.lib()
+    # negative count
+    assert L.LibShalom_sgemm_batched_plan(ct.byref(h), 0, 0, None, None, None, None, None, None, -1, 0) == 1
+    # null arrays with count > 0
+    assert L.LibShalom_sgemm_batched_plan(ct.byref(h), 0, 0, None, None, None, None, None, None, 3, 0) == 2
+    # bad precision
+    assert L.LibShalom_sgemm_batched_plan(ct.byref(h), 0, 0, None, None, None, None, None, None, 0, 11) == 1
+    assert L.LibShalom_plan_execute(None, None) == 2
+    assert L.LibShalom_plan_destroy(None) == 0
+
+
+SHAPES = [(80, 80, 80), (8, 8, 8), (128, 128, 128), (4096, 16, 1024), (65536, 64, 1024), (16, 65536, 256),
+          (64, 3136, 576), (512, 49, 4608), (1, 1, 1), (4096, 4096, 4096), (3, 100000, 7), (100000, 3, 7),
+          (256, 2048, 5000), (31999, 1024, 88)]
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3, ls.BF16, ls.BF16X6, ls.FP64])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_planner_invariants(prec, shape):
+    M, N, K = shape
+    for mp in (False, True):
+        p = ls.query_plan(prec, M, N, K, num_cu=256, mp=mp)
+        tm, tn = p["tile"]
+        assert (tm, tn) in {(32, 32), (64, 64), (128, 128), (128, 32), (128, 64), (32, 128)}
+        if prec == ls.FP64:
+            assert (tm, tn) != (128, 128)
+        ks, kp = p["ksplit"], p["kper"]
+        assert ks >= 1 and kp >= 1
+        assert ks * kp >= K and (ks - 1) * kp < max(K, 1)
+        if ks > 1:
+            assert kp % (16 if prec == ls.FP64 else 32) == 0
+            assert kp >= 4 * (16 if prec == ls.FP64 else 32)
+
+
+def test_planner_fills_the_chip():
+    # tall-skinny: few tiles, so the planner splits K
+    p = ls.query_plan(ls.FP32, 4096, 16, 1024, num_cu=256, mp=True)
+    tiles = -(-4096 // p["tile"][0]) * -(-16 // p["tile"][1])
+    assert tiles * p["ksplit"] >= 64
+    # big square: plenty of tiles, no split
+    p = ls.query_plan(ls.FP32, 4096, 4096, 4096, num_cu=256)
+    assert p["ksplit"] == 1
+
+
+def test_planner_errors():
+    with pytest.raises(ls.LibShalomError):
+        ls.query_plan(7, 4, 4, 4)
+    with pytest.raises(ls.LibShalomError):
+        ls.query_plan(ls.FP32, -1, 4, 4)
+    with pytest.raises(ls.LibShalomError):
+        ls.query_plan(ls.FP32, 4, 4, 4, num_cu=0)
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3, ls.FP64])
+def test_batched_tile_rule(prec):
+    assert ls.query_batched_tile(prec, 5, 32, 100) == (32, 32)
+    assert ls.query_batched_tile(prec, 33, 10, 1) == (64, 64)
+    big = ls.query_batched_tile(prec, 100, 128, 7)
+    assert big == ((64, 64) if prec == ls.FP64 else (128, 128))
