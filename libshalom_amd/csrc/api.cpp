@@ -259,11 +259,49 @@ struct LibShalom_plan_s {
 
 namespace {
 
+// How a batched plan executes its buckets.
+enum BatchMode : int {
+    BM_PERSIST = 0,  // one persistent launch per tile bucket, jobs streamed across
+    BM_GRID = 1,     // one workgroup per (GEMM, tile) work item
+    BM_SINGLE = 2,   // every GEMM in the largest square-tile bucket: one persistent launch
+};
+
+int batch_mode_from_env()
+{
+    const char *m = std::getenv("LIBSHALOM_BATCHED_MODE");
+    if (!m) return BM_PERSIST;
+    if (!std::strcmp(m, "grid")) return BM_GRID;
+    if (!std::strcmp(m, "single")) return BM_SINGLE;
+    return BM_PERSIST;
+}
+
+// A tile of a GEMM as a GEMM of its own: pointers offset to the tile, M and N
+// cut to the tile, alignment flags recomputed.
+GemmDesc tile_job(int ta, int tb, const GemmDesc &g, int m0, int n0, int bm, int bn, int esize)
+{
+    GemmDesc j = g;
+    const char *A = (const char *)g.A, *B = (const char *)g.B;
+    char *C = (char *)g.C;
+    j.A = A + (size_t)esize * (ta ? (size_t)m0 : (size_t)m0 * g.lda);
+    j.B = B + (size_t)esize * (tb ? (size_t)n0 * g.ldb : (size_t)n0);
+    j.C = C + (size_t)esize * ((size_t)m0 * g.ldc + n0);
+    j.M = std::min(bm, g.M - m0);
+    j.N = std::min(bn, g.N - n0);
+    const int vec = 16 / esize;
+    j.flags = 0;
+    if (((uintptr_t)j.A % 16) == 0 && j.lda % vec == 0) j.flags |= 1;
+    if (((uintptr_t)j.B % 16) == 0 && j.ldb % vec == 0) j.flags |= 2;
+    return j;
+}
+
 struct Bucket {
     int tile;
-    std::vector<GemmDesc> descs;
-    std::vector<WorkItem> items;
+    std::vector<GemmDesc> descs;  // GEMMs (grid mode) or tile jobs (persistent modes)
+    std::vector<WorkItem> items;  // grid mode only
+    int grid = 0;                 // persistent launch size
 };
+
+double job_cost(const GemmDesc &d) { return (double)d.M * d.N * std::max(d.K, 1); }
 
 int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, const void *const *A,
                const void *const *B, const long *M, const long *N, const long *K, long count)
@@ -279,6 +317,8 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     if (st) return st;
     const bool f64 = prec == P_FP64;
     const int esize = f64 ? 8 : 4;
+    const int mode = batch_mode_from_env();
+    const int single_tile = f64 ? T64x64 : T128x128;
 
     std::vector<Bucket> buckets(NTILE);
     for (int t = 0; t < NTILE; ++t) buckets[t].tile = t;
@@ -291,23 +331,39 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
         if (M[g] == 0 || N[g] == 0) continue;
         flops += 2.0 * M[g] * N[g] * K[g];
         bytes += (double)esize * (M[g] * K[g] + K[g] * N[g] + M[g] * N[g]);
-        const int tile = batched_tile(prec, M[g], N[g], K[g]);
-        buckets[tile].descs.push_back(make_desc(ta, tb, A[g], B[g], C[g], M[g], N[g], K[g], esize));
+        const int tile = mode == BM_SINGLE ? single_tile : batched_tile(prec, M[g], N[g], K[g]);
+        const GemmDesc d = make_desc(ta, tb, A[g], B[g], C[g], M[g], N[g], K[g], esize);
+        Bucket &b = buckets[tile];
+        if (mode == BM_GRID) {
+            b.descs.push_back(d);
+        } else {
+            const TileShape ts = kTileShapes[tile];
+            for (int m0 = 0; m0 < d.M; m0 += ts.bm)
+                for (int n0 = 0; n0 < d.N; n0 += ts.bn) b.descs.push_back(tile_job(ta, tb, d, m0, n0, ts.bm, ts.bn, esize));
+        }
     }
-    // Longest GEMMs first inside a bucket; biggest tiles first across buckets.
+    // Longest work first inside a bucket (persistent workgroups take jobs
+    // round-robin, so each gets a similar mix); biggest tiles first across buckets.
     long total_items = 0;
     for (auto &b : buckets) {
-        std::stable_sort(b.descs.begin(), b.descs.end(), [](const GemmDesc &x, const GemmDesc &y) {
-            return (double)x.M * x.N * x.K > (double)y.M * y.N * y.K;
-        });
-        const TileShape ts = kTileShapes[b.tile];
-        for (int i = 0; i < (int)b.descs.size(); ++i) {
-            const GemmDesc &d = b.descs[i];
-            const int nt = ((d.M + ts.bm - 1) / ts.bm) * ((d.N + ts.bn - 1) / ts.bn);
-            for (int t = 0; t < nt; ++t) b.items.push_back(WorkItem{i, t});
+        std::stable_sort(b.descs.begin(), b.descs.end(),
+                         [](const GemmDesc &x, const GemmDesc &y) { return job_cost(x) > job_cost(y); });
+        if (mode == BM_GRID) {
+            const TileShape ts = kTileShapes[b.tile];
+            for (int i = 0; i < (int)b.descs.size(); ++i) {
+                const GemmDesc &d = b.descs[i];
+                const int nt = ((d.M + ts.bm - 1) / ts.bm) * ((d.N + ts.bn - 1) / ts.bn);
+                for (int t = 0; t < nt; ++t) b.items.push_back(WorkItem{i, t});
+            }
+            total_items += (long)b.items.size();
+            if ((long)b.items.size() > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
+        } else if (!b.descs.empty()) {
+            total_items += (long)b.descs.size();
+            if ((long)b.descs.size() > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
+            PersistOccupancy occ = launch_table().persist_occ[prec][ta][tb][b.tile];
+            const int per_cu = occ ? std::max(1, occ()) : 1;
+            b.grid = (int)std::min<long>((long)b.descs.size(), (long)per_cu * info.num_cu);
         }
-        total_items += (long)b.items.size();
-        if ((long)b.items.size() > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
     }
 
     LibShalom_plan_s *plan = new (std::nothrow) LibShalom_plan_s();
@@ -355,11 +411,16 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     bool launch_ok = true;
     for (int oi = 0; oi < NTILE; ++oi) {
         const Bucket &b = buckets[order[oi]];
-        if (b.items.empty()) continue;
-        BatchedLaunch fn = launch_table().batched[prec][ta][tb][b.tile];
+        if (b.descs.empty()) continue;
         const GemmDesc *dd = (const GemmDesc *)((char *)plan->dev_buf + offs[b.tile].first);
-        const WorkItem *di = (const WorkItem *)((char *)plan->dev_buf + offs[b.tile].second);
-        if (!fn || fn(dd, di, (int)b.items.size(), cs) != hipSuccess) launch_ok = false;
+        if (mode == BM_GRID) {
+            BatchedLaunch fn = launch_table().batched[prec][ta][tb][b.tile];
+            const WorkItem *di = (const WorkItem *)((char *)plan->dev_buf + offs[b.tile].second);
+            if (!fn || fn(dd, di, (int)b.items.size(), cs) != hipSuccess) launch_ok = false;
+        } else {
+            PersistLaunch fn = launch_table().persist[prec][ta][tb][b.tile];
+            if (!fn || fn(dd, (int)b.descs.size(), b.grid, cs) != hipSuccess) launch_ok = false;
+        }
     }
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &g);

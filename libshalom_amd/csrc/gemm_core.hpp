@@ -106,6 +106,9 @@ struct TileCfg {
     static constexpr int BUF_BYTES = A_BYTES + B_BYTES;
     static constexpr int NBUF = (2 * BUF_BYTES <= 80 * 1024) ? 2 : 1;
     static constexpr int LDS_BYTES = NBUF * BUF_BYTES;
+    // Workgroups per CU the LDS allows (160 KB per CU); the persistent kernel
+    // asks the compiler to fit its registers to this residency.
+    static constexpr int WG_PER_CU = (160 * 1024 / LDS_BYTES) < 8 ? (160 * 1024 / LDS_BYTES) : 8;
     static constexpr int WM_MAX = BM / 32, WN_MAX = BN / 32;  // sub-tiles per wave (2x2 grid fits)
     static constexpr bool A_KCONTIG = !TA;  // A[m][k] rows contiguous along k
     static constexpr bool B_KCONTIG = TB;   // B^T[n][k] contiguous along k
@@ -403,46 +406,126 @@ struct Tile {
             st.template store_split<Cfg::NP, Cfg::LDKH>((__bf16 *)dst, tid);
     }
 
+    // Wave layout for a tile of d at (m0, n0), accumulators cleared.
+    __device__ __forceinline__ void begin(int M, int N, int m0, int n0, int wave)
+    {
+        layout((min(BM, M - m0) + 15) >> 4, (min(BN, N - n0) + 15) >> 4, wave);
+        zero();
+    }
+
+    // Global -> registers: the K chunk at k0 of the tile at (m0, n0).
+    __device__ __forceinline__ static void load_chunk(StageA &sa, StageB &sb, const GemmDesc &d, int m0, int n0,
+                                                      int k0, int kend, int tid)
+    {
+        sa.load((const T *)d.A, d.lda, d.M, kend, m0, k0, d.flags & 1, tid);
+        sb.load((const T *)d.B, d.ldb, d.N, kend, n0, k0, d.flags & 2, tid);
+    }
+
+    // Registers -> LDS in the packed MFMA operand layout (transpose / split).
+    __device__ __forceinline__ static void chunk_to_lds(const StageA &sa, const StageB &sb, char *buf, int tid)
+    {
+        store_stage(sa, buf, tid);
+        store_stage(sb, buf + Cfg::A_BYTES, tid);
+    }
+
+    __device__ __forceinline__ static int ksteps(int kend, int k0) { return min(Cfg::NSLOT, (kend - k0 + 3) >> 2); }
+
+    __device__ __forceinline__ static char *lds_buf(char *lds, int i)
+    {
+        return lds + (Cfg::NBUF == 2 ? (i & 1) * Cfg::BUF_BYTES : 0);
+    }
+
     // Run the K loop over [kbeg, kend) for the tile at (m0, n0) of d.
     __device__ __forceinline__ void run(const GemmDesc &d, int m0, int n0, int kbeg, int kend, char *lds)
     {
         const int tid = threadIdx.x;
         const int lane = tid & (kWave - 1);
         const int wave = uniform(tid >> 6);
-        const int mt = (min(BM, d.M - m0) + 15) >> 4;
-        const int nt = (min(BN, d.N - n0) + 15) >> 4;
-        layout(mt, nt, wave);
-        zero();
+        begin(d.M, d.N, m0, n0, wave);
         if (kend <= kbeg) return;
-        const T *A = (const T *)d.A;
-        const T *B = (const T *)d.B;
-        const bool va = d.flags & 1, vb = d.flags & 2;
         const int nchunks = (kend - kbeg + BK - 1) / BK;
         StageA sa;
         StageB sb;
-        sa.load(A, d.lda, d.M, kend, m0, kbeg, va, tid);
-        sb.load(B, d.ldb, d.N, kend, n0, kbeg, vb, tid);
-        store_stage(sa, lds, tid);
-        store_stage(sb, lds + Cfg::A_BYTES, tid);
+        load_chunk(sa, sb, d, m0, n0, kbeg, kend, tid);
+        chunk_to_lds(sa, sb, lds, tid);
         __syncthreads();
         for (int c = 0; c < nchunks; ++c) {
             const int k0 = kbeg + c * BK;
             const bool more = c + 1 < nchunks;
-            char *cur = lds + (Cfg::NBUF == 2 ? (c & 1) * Cfg::BUF_BYTES : 0);
+            if (more) load_chunk(sa, sb, d, m0, n0, k0 + BK, kend, tid);
+            compute(lds_buf(lds, c), lane, ksteps(kend, k0));
             if (more) {
-                sa.load(A, d.lda, d.M, kend, m0, k0 + BK, va, tid);
-                sb.load(B, d.ldb, d.N, kend, n0, k0 + BK, vb, tid);
-            }
-            const int ksteps = min(Cfg::NSLOT, (kend - k0 + 3) >> 2);
-            compute(cur, lane, ksteps);
-            if (more) {
-                char *nxt = lds + (Cfg::NBUF == 2 ? ((c + 1) & 1) * Cfg::BUF_BYTES : 0);
                 if constexpr (Cfg::NBUF == 1) __syncthreads();
-                store_stage(sa, nxt, tid);
-                store_stage(sb, nxt + Cfg::A_BYTES, tid);
+                chunk_to_lds(sa, sb, lds_buf(lds, c + 1), tid);
             }
             __syncthreads();
         }
+    }
+
+    // Persistent job stream: this workgroup takes jobs[first], jobs[first +
+    // stride], ...  Each job is a GEMM of at most BM x BN whose pointers are
+    // already offset to the tile.  The stream of K chunks runs across job
+    // boundaries: the first chunk of the next job is loaded while the last
+    // chunk of the current one is multiplied and its C tile written, so the
+    // loads stay in flight instead of draining at every job.
+    __device__ __forceinline__ void stream_jobs(const GemmDesc *__restrict__ jobs, int njobs, int first, int stride,
+                                                char *lds)
+    {
+        const int tid = threadIdx.x;
+        const int lane = tid & (kWave - 1);
+        const int wave = uniform(tid >> 6);
+        int j = first;
+        if (j >= njobs) return;
+        GemmDesc d = jobs[j];
+        int nj = j + stride;
+        GemmDesc nd = d;
+        if (nj < njobs) nd = jobs[nj];
+        begin(d.M, d.N, 0, 0, wave);
+        int nch = max(1, (d.K + BK - 1) / BK), c = 0, buf = 0;
+        StageA sa;
+        StageB sb;
+        load_chunk(sa, sb, d, 0, 0, 0, d.K, tid);
+        chunk_to_lds(sa, sb, lds, tid);
+        __syncthreads();
+        for (;;) {
+            const bool last = c + 1 == nch;
+            const bool more = !last || nj < njobs;
+            if (more) {
+                if (!last)
+                    load_chunk(sa, sb, d, 0, 0, (c + 1) * BK, d.K, tid);
+                else
+                    load_chunk(sa, sb, nd, 0, 0, 0, nd.K, tid);
+            }
+            compute(lds_buf(lds, buf), lane, ksteps(d.K, c * BK));
+            if (last) store((T *)d.C, d.ldc, d.M, d.N, 0, 0);
+            if (!more) break;
+            if (last) {
+                d = nd;
+                nj += stride;
+                if (nj < njobs) nd = jobs[nj];
+                begin(d.M, d.N, 0, 0, wave);
+                nch = max(1, (d.K + BK - 1) / BK);
+                c = 0;
+            } else {
+                ++c;
+            }
+            if constexpr (Cfg::NBUF == 1) __syncthreads();
+            chunk_to_lds(sa, sb, lds_buf(lds, buf + 1), tid);
+            __syncthreads();
+            ++buf;
+        }
+    }
+
+    // Row of the 16x16 sub-tile held in accumulator register i of lane group
+    // q.  v_mfma_f32_16x16x4_f32 gives lane (r, q) rows 4q..4q+3;
+    // v_mfma_f64_16x16x4_f64 interleaves them: rows q, q+4, q+8, q+12
+    // (column r in both; measured on gfx950).
+    __device__ __forceinline__ static int acc_row(int q, int i)
+    {
+        if constexpr (sizeof(T) == 8)
+            return q + 4 * i;
+        else
+            return 4 * q + i;
     }
 
     // Store the accumulators to a row-major OutT matrix (C or a split-K
@@ -458,11 +541,13 @@ struct Tile {
             for (int b = 0; b < WN; ++b)
                 if (a < wm && b < wn) {
                     const int col = n0 + (wc0 + b) * 16 + r;
-                    const int rowb = m0 + (wr0 + a) * 16 + 4 * q;
+                    const int row0 = m0 + (wr0 + a) * 16;
                     if (col < N) {
 #pragma unroll
-                        for (int i = 0; i < 4; ++i)
-                            if (rowb + i < M) out[(size_t)(rowb + i) * ldo + col] = (OutT)acc[a][b][i];
+                        for (int i = 0; i < 4; ++i) {
+                            const int row = row0 + acc_row(q, i);
+                            if (row < M) out[(size_t)row * ldo + col] = (OutT)acc[a][b][i];
+                        }
                     }
                 }
     }

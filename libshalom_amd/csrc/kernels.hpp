@@ -50,6 +50,18 @@ __global__ __launch_bounds__(kThreads) void k_gemm_batched(const GemmDesc *__res
     t.store((typename Cfg::T *)d.C, d.ldc, d.M, d.N, m0, n0);
 }
 
+// Persistent executor: gridDim.x workgroups (as many as can be resident)
+// stream a list of tile jobs, workgroup b taking jobs b, b + grid, ...
+template <class Cfg>
+__global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+                          amdgpu_waves_per_eu(Cfg::WG_PER_CU, 8))) void
+k_gemm_persistent(const GemmDesc *__restrict__ jobs, int njobs)
+{
+    __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
+    Tile<Cfg> t;
+    t.stream_jobs(jobs, njobs, blockIdx.x, gridDim.x, lds);
+}
+
 // C = sum over slices of partial[s], in slice order.
 template <typename T>
 __global__ __launch_bounds__(256) void k_splitk_reduce(const T *__restrict__ partial, T *__restrict__ C,
@@ -67,10 +79,14 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const T *__restrict__ par
 // ---------------------------------------------------------------------------
 typedef hipError_t (*TilesLaunch)(const GemmDesc &d, int ksplit, int kper, void *partial, hipStream_t s);
 typedef hipError_t (*BatchedLaunch)(const GemmDesc *descs, const WorkItem *items, int nitems, hipStream_t s);
+typedef hipError_t (*PersistLaunch)(const GemmDesc *jobs, int njobs, int grid, hipStream_t s);
+typedef int (*PersistOccupancy)();  // resident workgroups per CU, 0 on error
 
 struct LaunchTable {
     TilesLaunch tiles[NPREC][2][2][NTILE];
     BatchedLaunch batched[NPREC][2][2][NTILE];
+    PersistLaunch persist[NPREC][2][2][NTILE];
+    PersistOccupancy persist_occ[NPREC][2][2][NTILE];
 };
 LaunchTable &launch_table();
 
@@ -92,12 +108,35 @@ hipError_t launch_batched(const GemmDesc *descs, const WorkItem *items, int nite
     return hipGetLastError();
 }
 
+template <class Cfg>
+hipError_t launch_persistent(const GemmDesc *jobs, int njobs, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gemm_persistent<Cfg>, dim3(grid), dim3(kThreads), 0, s, jobs, njobs);
+    return hipGetLastError();
+}
+
+template <class Cfg>
+int persistent_occupancy()
+{
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_gemm_persistent<Cfg>, kThreads, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
 template <int PREC, int TA, int TB, int BM, int BN>
 void register_one(LaunchTable &t, int tile)
 {
     typedef TileCfg<PREC, TA, TB, BM, BN> Cfg;
     t.tiles[PREC][TA][TB][tile] = &launch_tiles<Cfg>;
     t.batched[PREC][TA][TB][tile] = &launch_batched<Cfg>;
+    // the batched buckets use the square tiles
+    if (BM == BN) {
+        t.persist[PREC][TA][TB][tile] = &launch_persistent<Cfg>;
+        t.persist_occ[PREC][TA][TB][tile] = &persistent_occupancy<Cfg>;
+    }
 }
 
 template <int PREC, int TA, int TB>
