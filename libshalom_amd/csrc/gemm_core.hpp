@@ -23,8 +23,10 @@
 // * Register tiling.  Each wave owns a rectangle of 16x16 sub-tiles (at most
 //   BM/32 x BN/32); the wave grid (2x2, 1x4 or 4x1) is chosen per tile at run
 //   time so small or skinny tiles keep all four waves busy.
-// * Pipelining.  The next K chunk is loaded into registers while the current
-//   one is multiplied out of LDS; LDS is double-buffered when it fits.
+// * Pipelining.  A workgroup runs a stream of jobs (tiles of GEMMs) as one
+//   flat sequence of K chunks: two chunks are in flight in registers while
+//   the current one is multiplied out of LDS, across job boundaries; LDS is
+//   double-buffered when it fits.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -86,6 +88,35 @@ __device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[3])
 }
 
 // ---------------------------------------------------------------------------
+// LDS panel layouts of one operand chunk (ROWS x BK; "rows" are M for A, N
+// for B).  Which one is used depends on the mode and on whether the operand
+// is contiguous along k in memory (A NoTrans, B Trans) or along rows (A
+// Trans, B NoTrans).  Each is chosen so both the store from registers and
+// the MFMA operand reads stay (nearly) free of LDS bank conflicts.
+//   ROWK  FP32/FP64, k-contiguous operand: [row][BK + pad], k permuted (kpos)
+//         so a lane's k steps are one contiguous 16-byte run.
+//   KROW  FP32/FP64, row-contiguous operand: [k][ROWS + 16], a straight copy
+//         of the 16-byte global loads; the MFMA reads one element per k step.
+//   SPLIT BF16 modes: NP planes of [row][BK + 8] bf16, natural k order; the
+//         threads hold four consecutive k of one row (16-byte loads for a
+//         k-contiguous operand, lane-per-row 4-byte loads otherwise) and write
+//         one 8-byte run per plane.
+// ---------------------------------------------------------------------------
+enum Layout : int { L_ROWK = 0, L_KROW = 1, L_SPLIT = 2 };
+
+template <typename T, int NP, int ROWS, int BK, bool KCONTIG>
+struct PanelInfo {
+    static constexpr int LAYOUT = NP > 0 ? L_SPLIT : (KCONTIG ? L_ROWK : L_KROW);
+    static constexpr int VEC = 16 / sizeof(T);
+    static constexpr int LDK = BK + VEC;    // ROWK row length (elements)
+    static constexpr int LDR = ROWS + 16;   // KROW row length (elements)
+    static constexpr int LDKH = BK + 8;     // SPLIT row length (bf16)
+    static constexpr int BYTES = LAYOUT == L_SPLIT ? NP * ROWS * LDKH * 2
+                                 : LAYOUT == L_ROWK ? ROWS * LDK * (int)sizeof(T)
+                                                    : BK * LDR * (int)sizeof(T);
+};
+
+// ---------------------------------------------------------------------------
 // Configuration of one tile kernel.
 // ---------------------------------------------------------------------------
 template <int PREC_, int TA_, int TB_, int BM_, int BN_>
@@ -97,27 +128,24 @@ struct TileCfg {
     static constexpr int BK = sizeof(T) == 8 ? 16 : 32;
     static constexpr int VEC = 16 / sizeof(T);            // elements per 16-byte load
     static constexpr int NSLOT = BK / 4;                  // 16x16x4 MFMA k-steps per chunk
-    // LDS panels.  Unsplit: [rows][LDK] of T, k position permuted (see kpos).
-    // Split: [plane][rows][LDKH] of bf16, natural k order.
-    static constexpr int LDK = BK + VEC;                  // 16-byte row pad
-    static constexpr int LDKH = BK + 8;                   // 16-byte row pad (bf16)
-    static constexpr int A_BYTES = SPLIT ? NP * BM * LDKH * 2 : BM * LDK * (int)sizeof(T);
-    static constexpr int B_BYTES = SPLIT ? NP * BN * LDKH * 2 : BN * LDK * (int)sizeof(T);
+    static constexpr bool A_KCONTIG = !TA;  // A[m][k] rows contiguous along k
+    static constexpr bool B_KCONTIG = TB;   // B^T[n][k] contiguous along k
+    typedef PanelInfo<T, NP, BM, BK, A_KCONTIG> PA;
+    typedef PanelInfo<T, NP, BN, BK, B_KCONTIG> PB;
+    static constexpr int LDKH = BK + 8;
+    static constexpr int A_BYTES = PA::BYTES;
+    static constexpr int B_BYTES = PB::BYTES;
     static constexpr int BUF_BYTES = A_BYTES + B_BYTES;
     static constexpr int NBUF = (2 * BUF_BYTES <= 80 * 1024) ? 2 : 1;
     static constexpr int LDS_BYTES = NBUF * BUF_BYTES;
-    // Workgroups per CU the LDS allows (160 KB per CU); the persistent kernel
-    // asks the compiler to fit its registers to this residency.
-    static constexpr int WG_PER_CU = (160 * 1024 / LDS_BYTES) < 8 ? (160 * 1024 / LDS_BYTES) : 8;
     static constexpr int WM_MAX = BM / 32, WN_MAX = BN / 32;  // sub-tiles per wave (2x2 grid fits)
-    static constexpr bool A_KCONTIG = !TA;  // A[m][k] rows contiguous along k
-    static constexpr bool B_KCONTIG = TB;   // B^T[n][k] contiguous along k
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
     static_assert(WM_MAX * WN_MAX <= 16, "at most 16 sub-tiles per wave");
+    static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "panels stay 16-byte aligned");
 };
 
-// LDS position of k inside a chunk for the unsplit layout: lane group q of a
+// LDS position of k inside a chunk for the ROWK layout: lane group q of a
 // 16x16x4 MFMA takes k = 4*s + q at step s, and a lane reads its NSLOT values
 // for steps 0..NSLOT-1 as one contiguous run, so k = 4*s + q is stored at
 // q*NSLOT + s.  Steps whose k are all >= K can then be skipped.
@@ -126,21 +154,28 @@ __device__ __forceinline__ int kpos(int k) { return (k & 3) * NSLOT + (k >> 2); 
 
 // ---------------------------------------------------------------------------
 // Global -> register staging of one operand panel chunk (ROWS x BK).
-// "rows" are M for A and N for B.  KCONTIG: storage is [row][k] (stride ld);
-// otherwise [k][row].
+// KCONTIG: storage is [row][k] (stride ld); otherwise [k][row].  LANEROW
+// (BF16 modes, row-contiguous operand): lane-per-row 4-byte loads, each thread
+// holding VEC consecutive k of one row.
 // ---------------------------------------------------------------------------
-template <typename T, int ROWS, int BK, bool KCONTIG>
+template <typename T, int NP, int ROWS, int BK, bool KCONTIG>
 struct Stage {
+    typedef PanelInfo<T, NP, ROWS, BK, KCONTIG> P;
     static constexpr int VEC = 16 / sizeof(T);
+    static constexpr bool LANEROW = NP > 0 && !KCONTIG;
     static constexpr int NVEC = ROWS * BK / VEC;
     static constexpr int PER = (NVEC + kThreads - 1) / kThreads;
     T v[PER][VEC];
 
+    // First element (row, k) of the thread's group of VEC elements.
     __device__ __forceinline__ static void coords(int idx, int &row, int &k)
     {
         if constexpr (KCONTIG) {
             row = idx / (BK / VEC);
             k = (idx % (BK / VEC)) * VEC;
+        } else if constexpr (LANEROW) {
+            row = idx % ROWS;
+            k = (idx / ROWS) * VEC;
         } else {
             k = idx / (ROWS / VEC);
             row = (idx % (ROWS / VEC)) * VEC;
@@ -178,6 +213,13 @@ struct Stage {
                             if (gk + e < kend) v[i][e] = p[e];
                     }
                 }
+            } else if constexpr (LANEROW) {
+                if (active && gr < R) {
+                    const T *p = base + (size_t)gk * ld + gr;
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e)
+                        if (gk + e < kend) v[i][e] = p[(size_t)e * ld];
+                }
             } else {
                 if (active && gk < kend) {
                     const T *p = base + (size_t)gk * ld + gr;
@@ -201,9 +243,8 @@ struct Stage {
         }
     }
 
-    // Unsplit LDS layout: [ROWS][LDK] with permuted k.
-    template <int LDK>
-    __device__ __forceinline__ void store_plain(T *__restrict__ lds, int tid) const
+    // Registers -> LDS panel.
+    __device__ __forceinline__ void store(char *__restrict__ dst, int tid) const
     {
         constexpr int NSLOT = BK / 4;
 #pragma unroll
@@ -212,43 +253,29 @@ struct Stage {
             if (!((NVEC % kThreads == 0) || idx < NVEC)) continue;
             int row, k;
             coords(idx, row, k);
+            if constexpr (P::LAYOUT == L_ROWK) {
+                T *lds = (T *)dst;
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-                if constexpr (KCONTIG)
-                    lds[row * LDK + kpos<NSLOT>(k + e)] = v[i][e];
+                for (int e = 0; e < VEC; ++e) lds[row * P::LDK + kpos<NSLOT>(k + e)] = v[i][e];
+            } else if constexpr (P::LAYOUT == L_KROW) {
+                T *lds = (T *)dst + k * P::LDR + row;
+                if constexpr (sizeof(T) == 4)
+                    *(f32x4 *)lds = f32x4{v[i][0], v[i][1], v[i][2], v[i][3]};
                 else
-                    lds[(row + e) * LDK + kpos<NSLOT>(k)] = v[i][e];
-            }
-        }
-    }
-
-    // Split LDS layout: NP planes of [ROWS][LDKH] bf16, natural k order.
-    template <int NP, int LDKH>
-    __device__ __forceinline__ void store_split(__bf16 *__restrict__ lds, int tid) const
-    {
-        static_assert(sizeof(T) == 4, "split modes take FP32 inputs");
-        constexpr int PLANE = ROWS * LDKH;
+                    *(f64x2 *)lds = f64x2{v[i][0], v[i][1]};
+            } else {
+                static_assert(sizeof(T) == 4, "split modes take FP32 inputs");
+                __bf16 *lds = (__bf16 *)dst;
+                constexpr int PLANE = ROWS * P::LDKH;
+                __bf16 p[VEC][3];
 #pragma unroll
-        for (int i = 0; i < PER; ++i) {
-            const int idx = tid + i * kThreads;
-            if (!((NVEC % kThreads == 0) || idx < NVEC)) continue;
-            int row, k;
-            coords(idx, row, k);
-            __bf16 p[VEC][3];
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) split_bf16<NP>(v[i][e], p[e]);
-            if constexpr (KCONTIG) {
+                for (int e = 0; e < VEC; ++e) split_bf16<NP>(v[i][e], p[e]);
                 // four consecutive k of one row: one 8-byte write per plane
 #pragma unroll
                 for (int pl = 0; pl < NP; ++pl) {
                     bf16x4 w = {p[0][pl], p[1][pl], p[2][pl], p[3][pl]};
-                    *(bf16x4 *)(lds + pl * PLANE + row * LDKH + k) = w;
+                    *(bf16x4 *)(lds + pl * PLANE + row * P::LDKH + k) = w;
                 }
-            } else {
-#pragma unroll
-                for (int pl = 0; pl < NP; ++pl)
-#pragma unroll
-                    for (int e = 0; e < VEC; ++e) lds[pl * PLANE + (row + e) * LDKH + k] = p[e][pl];
             }
         }
     }
@@ -263,8 +290,8 @@ struct Tile {
     typedef typename Cfg::Acc Acc;
     static constexpr int BM = Cfg::BM, BN = Cfg::BN, BK = Cfg::BK;
     static constexpr int WM = Cfg::WM_MAX, WN = Cfg::WN_MAX;
-    typedef Stage<T, BM, BK, Cfg::A_KCONTIG> StageA;
-    typedef Stage<T, BN, BK, Cfg::B_KCONTIG> StageB;
+    typedef Stage<T, Cfg::NP, BM, BK, Cfg::A_KCONTIG> StageA;
+    typedef Stage<T, Cfg::NP, BN, BK, Cfg::B_KCONTIG> StageB;
 
     Acc acc[WM][WN];
     int wr0, wc0, wm, wn;  // this wave's sub-tile rectangle (in 16-row/col units)
@@ -303,107 +330,107 @@ struct Tile {
     }
 
     // Multiply one LDS chunk.  ksteps = number of valid 4-wide k steps (unsplit).
+    // The wave's A fragments stay in registers; B fragments are read one
+    // 16-column block at a time, and the MFMAs of one k step (or one split
+    // product) are issued across the independent row blocks so consecutive
+    // MFMAs do not wait on each other.
     __device__ __forceinline__ void compute(const char *__restrict__ buf, int lane, int ksteps)
     {
         const int r = lane & 15, q = lane >> 4;
         if constexpr (!Cfg::SPLIT) {
-            constexpr int NS = Cfg::NSLOT, LDK = Cfg::LDK;
+            constexpr int NS = Cfg::NSLOT;
             const T *As = (const T *)buf;
             const T *Bs = (const T *)(buf + Cfg::A_BYTES);
-            T af[WM][NS], bfr[WN][NS];
+            T af[WM][NS];
 #pragma unroll
-            for (int a = 0; a < WM; ++a) {
-                if (a < wm) {
-                    const T *p = As + ((wr0 + a) * 16 + r) * LDK + q * NS;
-#pragma unroll
-                    for (int s = 0; s < NS; s += 16 / sizeof(T)) {
-                        if constexpr (sizeof(T) == 4) {
-                            f32x4 t = *(const f32x4 *)(p + s);
-                            af[a][s] = t[0]; af[a][s + 1] = t[1]; af[a][s + 2] = t[2]; af[a][s + 3] = t[3];
-                        } else {
-                            f64x2 t = *(const f64x2 *)(p + s);
-                            af[a][s] = t[0]; af[a][s + 1] = t[1];
-                        }
-                    }
-                }
-            }
+            for (int a = 0; a < WM; ++a)
+                if (a < wm) read_frag<typename Cfg::PA>(As, wr0 + a, r, q, af[a]);
 #pragma unroll
             for (int b = 0; b < WN; ++b) {
                 if (b < wn) {
-                    const T *p = Bs + ((wc0 + b) * 16 + r) * LDK + q * NS;
+                    T bf[NS];
+                    read_frag<typename Cfg::PB>(Bs, wc0 + b, r, q, bf);
 #pragma unroll
-                    for (int s = 0; s < NS; s += 16 / sizeof(T)) {
-                        if constexpr (sizeof(T) == 4) {
-                            f32x4 t = *(const f32x4 *)(p + s);
-                            bfr[b][s] = t[0]; bfr[b][s + 1] = t[1]; bfr[b][s + 2] = t[2]; bfr[b][s + 3] = t[3];
-                        } else {
-                            f64x2 t = *(const f64x2 *)(p + s);
-                            bfr[b][s] = t[0]; bfr[b][s + 1] = t[1];
+                    for (int s = 0; s < NS; ++s) {
+                        if (s < ksteps) {
+#pragma unroll
+                            for (int a = 0; a < WM; ++a)
+                                if (a < wm) {
+                                    if constexpr (sizeof(T) == 4)
+                                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][s], bf[s], acc[a][b], 0, 0, 0);
+                                    else
+                                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a][s], bf[s], acc[a][b], 0, 0, 0);
+                                }
                         }
                     }
-                }
-            }
-#pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                if (s < ksteps) {
-#pragma unroll
-                    for (int a = 0; a < WM; ++a)
-#pragma unroll
-                        for (int b = 0; b < WN; ++b)
-                            if (a < wm && b < wn) {
-                                if constexpr (sizeof(T) == 4)
-                                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][s], bfr[b][s], acc[a][b], 0, 0, 0);
-                                else
-                                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a][s], bfr[b][s], acc[a][b], 0, 0, 0);
-                            }
                 }
             }
         } else {
             constexpr int NP = Cfg::NP, LDKH = Cfg::LDKH;
             const __bf16 *As = (const __bf16 *)buf;
             const __bf16 *Bs = (const __bf16 *)(buf + Cfg::A_BYTES);
-            bf16x8 af[WM][NP], bfr[WN][NP];
+            bf16x8 af[WM][NP];
 #pragma unroll
             for (int a = 0; a < WM; ++a)
                 if (a < wm)
 #pragma unroll
                     for (int pl = 0; pl < NP; ++pl)
                         af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + ((wr0 + a) * 16 + r) * LDKH + q * 8);
+            // partial products (A piece, B piece), smallest first
+            constexpr int NPROD = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
+            constexpr int PA[6] = {NP == 3 ? 2 : 1, NP == 3 ? 1 : 0, NP == 3 ? 0 : 0, 1, 0, 0};
+            constexpr int PB[6] = {0, NP == 3 ? 1 : 1, NP == 3 ? 2 : 0, 0, 1, 0};
 #pragma unroll
-            for (int b = 0; b < WN; ++b)
-                if (b < wn)
+            for (int b = 0; b < WN; ++b) {
+                if (b < wn) {
+                    bf16x8 bfr[NP];
 #pragma unroll
                     for (int pl = 0; pl < NP; ++pl)
-                        bfr[b][pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + ((wc0 + b) * 16 + r) * LDKH + q * 8);
+                        bfr[pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + ((wc0 + b) * 16 + r) * LDKH + q * 8);
 #pragma unroll
-            for (int a = 0; a < WM; ++a)
+                    for (int p = 0; p < NPROD; ++p) {
+                        const int ia = NP == 1 ? 0 : PA[p], ib = NP == 1 ? 0 : PB[p];
 #pragma unroll
-                for (int b = 0; b < WN; ++b)
-                    if (a < wm && b < wn) {
-                        Acc c = acc[a][b];
-                        // smallest partial products first
-                        if constexpr (NP >= 3) {
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][2], bfr[b][0], c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][1], bfr[b][1], c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bfr[b][2], c, 0, 0, 0);
-                        }
-                        if constexpr (NP >= 2) {
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][1], bfr[b][0], c, 0, 0, 0);
-                            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bfr[b][1], c, 0, 0, 0);
-                        }
-                        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][0], bfr[b][0], c, 0, 0, 0);
-                        acc[a][b] = c;
+                        for (int a = 0; a < WM; ++a)
+                            if (a < wm)
+                                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][ia], bfr[ib], acc[a][b], 0, 0, 0);
                     }
+                }
+            }
         }
     }
 
     template <class StageT>
     __device__ __forceinline__ static void store_stage(const StageT &st, char *dst, int tid)
     {
-        if constexpr (!Cfg::SPLIT)
-            st.template store_plain<Cfg::LDK>((T *)dst, tid);
-        else
-            st.template store_split<Cfg::NP, Cfg::LDKH>((__bf16 *)dst, tid);
+        st.store(dst, tid);
+    }
+
+    // The NS k-step values of one MFMA operand fragment: sub-tile row block
+    // `blk` (16 rows) of a ROWK or KROW panel, for lane (r, q).
+    template <class P>
+    __device__ __forceinline__ static void read_frag(const T *__restrict__ X, int blk, int r, int q,
+                                                     T (&f)[Cfg::NSLOT])
+    {
+        constexpr int NS = Cfg::NSLOT;
+        if constexpr (P::LAYOUT == L_ROWK) {
+            constexpr int V = 16 / sizeof(T);
+            const T *p = X + (blk * 16 + r) * P::LDK + q * NS;
+#pragma unroll
+            for (int s = 0; s < NS; s += V) {
+                if constexpr (sizeof(T) == 4) {
+                    f32x4 t = *(const f32x4 *)(p + s);
+                    f[s] = t[0]; f[s + 1] = t[1]; f[s + 2] = t[2]; f[s + 3] = t[3];
+                } else {
+                    f64x2 t = *(const f64x2 *)(p + s);
+                    f[s] = t[0]; f[s + 1] = t[1];
+                }
+            }
+        } else {
+            const T *p = X + q * P::LDR + blk * 16 + r;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) f[s] = p[4 * s * P::LDR];
+        }
     }
 
     // Wave layout for a tile of d at (m0, n0), accumulators cleared.
@@ -435,84 +462,89 @@ struct Tile {
         return lds + (Cfg::NBUF == 2 ? (i & 1) * Cfg::BUF_BYTES : 0);
     }
 
-    // Run the K loop over [kbeg, kend) for the tile at (m0, n0) of d.
-    __device__ __forceinline__ void run(const GemmDesc &d, int m0, int n0, int kbeg, int kend, char *lds)
+    __device__ __forceinline__ static int nchunks(int K) { return max(1, (K + BK - 1) / BK); }
+
+    // Position in the flat stream of K chunks of a workgroup's jobs.
+    struct Cursor {
+        GemmDesc d;  // the job (pointers offset to the tile, K from 0)
+        int job;     // job index, -1 past the end
+        int c;       // chunk within the job
+        int nch;     // chunks of the job
+    };
+
+    template <class Src>
+    __device__ __forceinline__ static void advance(Cursor &x, const Src &src, int stride)
     {
-        const int tid = threadIdx.x;
-        const int lane = tid & (kWave - 1);
-        const int wave = uniform(tid >> 6);
-        begin(d.M, d.N, m0, n0, wave);
-        if (kend <= kbeg) return;
-        const int nchunks = (kend - kbeg + BK - 1) / BK;
-        StageA sa;
-        StageB sb;
-        load_chunk(sa, sb, d, m0, n0, kbeg, kend, tid);
-        chunk_to_lds(sa, sb, lds, tid);
-        __syncthreads();
-        for (int c = 0; c < nchunks; ++c) {
-            const int k0 = kbeg + c * BK;
-            const bool more = c + 1 < nchunks;
-            if (more) load_chunk(sa, sb, d, m0, n0, k0 + BK, kend, tid);
-            compute(lds_buf(lds, c), lane, ksteps(kend, k0));
-            if (more) {
-                if constexpr (Cfg::NBUF == 1) __syncthreads();
-                chunk_to_lds(sa, sb, lds_buf(lds, c + 1), tid);
-            }
-            __syncthreads();
+        if (x.job < 0) return;
+        if (x.c + 1 < x.nch) {
+            ++x.c;
+            return;
+        }
+        const int nj = x.job + stride;
+        if (nj < src.count()) {
+            x.job = nj;
+            x.d = src.get(nj);
+            x.c = 0;
+            x.nch = nchunks(x.d.K);
+        } else {
+            x.job = -1;
         }
     }
 
-    // Persistent job stream: this workgroup takes jobs[first], jobs[first +
-    // stride], ...  Each job is a GEMM of at most BM x BN whose pointers are
-    // already offset to the tile.  The stream of K chunks runs across job
-    // boundaries: the first chunk of the next job is loaded while the last
-    // chunk of the current one is multiplied and its C tile written, so the
-    // loads stay in flight instead of draining at every job.
-    __device__ __forceinline__ void stream_jobs(const GemmDesc *__restrict__ jobs, int njobs, int first, int stride,
-                                                char *lds)
+    // One step of the chunk pipeline.  `cur` is in LDS buffer bi; its
+    // successor n1 is in flight into stage (a1, b1); stage (a0, b0) is free and
+    // receives n2.  Two chunks stay in flight while `cur` is multiplied.
+    template <class Src>
+    __device__ __forceinline__ bool step(Cursor &cur, Cursor &n1, Cursor &n2, StageA &a0, StageB &b0, StageA &a1,
+                                         StageB &b1, char *lds, int bi, const Src &src, int stride, int tid,
+                                         int lane, int wave)
+    {
+        if (n2.job >= 0) load_chunk(a0, b0, n2.d, 0, 0, n2.c * BK, n2.d.K, tid);
+        compute(lds_buf(lds, bi), lane, ksteps(cur.d.K, cur.c * BK));
+        if (cur.c + 1 == cur.nch) store((T *)cur.d.C, cur.d.ldc, cur.d.M, cur.d.N, 0, 0);
+        if (n1.job < 0) return false;
+        if (n1.c == 0) begin(n1.d.M, n1.d.N, 0, 0, wave);
+        if constexpr (Cfg::NBUF == 1) __syncthreads();
+        chunk_to_lds(a1, b1, lds_buf(lds, bi + 1), tid);
+        __syncthreads();
+        cur = n1;
+        n1 = n2;
+        advance(n2, src, stride);
+        return true;
+    }
+
+    // Job stream: this workgroup takes jobs first, first + stride, ... of src.
+    // Each job is a GEMM of at most BM x BN whose pointers are already offset
+    // to its tile (and K slice).  The K chunks of consecutive jobs form one
+    // stream: the chunks of the next job are loaded while the current job's
+    // last chunks are multiplied and its C tile written, so loads stay in
+    // flight instead of draining at every job.
+    template <class Src>
+    __device__ __forceinline__ void stream(const Src &src, int first, int stride, char *lds)
     {
         const int tid = threadIdx.x;
         const int lane = tid & (kWave - 1);
         const int wave = uniform(tid >> 6);
-        int j = first;
-        if (j >= njobs) return;
-        GemmDesc d = jobs[j];
-        int nj = j + stride;
-        GemmDesc nd = d;
-        if (nj < njobs) nd = jobs[nj];
-        begin(d.M, d.N, 0, 0, wave);
-        int nch = max(1, (d.K + BK - 1) / BK), c = 0, buf = 0;
-        StageA sa;
-        StageB sb;
-        load_chunk(sa, sb, d, 0, 0, 0, d.K, tid);
-        chunk_to_lds(sa, sb, lds, tid);
+        if (first >= src.count()) return;
+        Cursor cur;
+        cur.job = first;
+        cur.d = src.get(first);
+        cur.c = 0;
+        cur.nch = nchunks(cur.d.K);
+        Cursor n1 = cur;
+        advance(n1, src, stride);
+        Cursor n2 = n1;
+        advance(n2, src, stride);
+        StageA a0, a1;
+        StageB b0, b1;
+        load_chunk(a0, b0, cur.d, 0, 0, 0, cur.d.K, tid);
+        if (n1.job >= 0) load_chunk(a1, b1, n1.d, 0, 0, n1.c * BK, n1.d.K, tid);
+        begin(cur.d.M, cur.d.N, 0, 0, wave);
+        chunk_to_lds(a0, b0, lds_buf(lds, 0), tid);
         __syncthreads();
         for (;;) {
-            const bool last = c + 1 == nch;
-            const bool more = !last || nj < njobs;
-            if (more) {
-                if (!last)
-                    load_chunk(sa, sb, d, 0, 0, (c + 1) * BK, d.K, tid);
-                else
-                    load_chunk(sa, sb, nd, 0, 0, 0, nd.K, tid);
-            }
-            compute(lds_buf(lds, buf), lane, ksteps(d.K, c * BK));
-            if (last) store((T *)d.C, d.ldc, d.M, d.N, 0, 0);
-            if (!more) break;
-            if (last) {
-                d = nd;
-                nj += stride;
-                if (nj < njobs) nd = jobs[nj];
-                begin(d.M, d.N, 0, 0, wave);
-                nch = max(1, (d.K + BK - 1) / BK);
-                c = 0;
-            } else {
-                ++c;
-            }
-            if constexpr (Cfg::NBUF == 1) __syncthreads();
-            chunk_to_lds(sa, sb, lds_buf(lds, buf + 1), tid);
-            __syncthreads();
-            ++buf;
+            if (!step(cur, n1, n2, a0, b0, a1, b1, lds, 0, src, stride, tid, lane, wave)) break;
+            if (!step(cur, n1, n2, a1, b1, a0, b0, lds, 1, src, stride, tid, lane, wave)) break;
         }
     }
 
@@ -552,6 +584,47 @@ struct Tile {
                 }
     }
 };
+
+// Job source over a device array of jobs.
+struct ArraySrc {
+    const GemmDesc *__restrict__ jobs;
+    int n;
+    __device__ __forceinline__ int count() const { return n; }
+    __device__ __forceinline__ GemmDesc get(int j) const { return jobs[j]; }
+};
+
+// Job source of exactly one job held in registers.
+struct OneSrc {
+    GemmDesc job;
+    __device__ __forceinline__ int count() const { return 1; }
+    __device__ __forceinline__ GemmDesc get(int) const { return job; }
+};
+
+// The tile (m0, n0) and K slice [kbeg, kend) of g as a job of its own, its
+// C tile written to out (leading dimension ldo) at (m0, n0).
+template <class Cfg>
+__device__ __forceinline__ GemmDesc tile_job(const GemmDesc &g, int m0, int n0, int kbeg, int kend, void *out,
+                                             int ldo)
+{
+    typedef typename Cfg::T T;
+    GemmDesc j;
+    const T *A = (const T *)g.A;
+    const T *B = (const T *)g.B;
+    j.A = A + (Cfg::TA ? (size_t)kbeg * g.lda + m0 : (size_t)m0 * g.lda + kbeg);
+    j.B = B + (Cfg::TB ? (size_t)n0 * g.ldb + kbeg : (size_t)kbeg * g.ldb + n0);
+    j.C = (T *)out + (size_t)m0 * ldo + n0;
+    j.M = min(Cfg::BM, g.M - m0);
+    j.N = min(Cfg::BN, g.N - n0);
+    j.K = max(0, kend - kbeg);
+    j.lda = g.lda;
+    j.ldb = g.ldb;
+    j.ldc = ldo;
+    int f = 0;
+    if ((((uintptr_t)j.A) & 15) == 0 && g.lda % Cfg::VEC == 0) f |= 1;
+    if ((((uintptr_t)j.B) & 15) == 0 && g.ldb % Cfg::VEC == 0) f |= 2;
+    j.flags = f;
+    return j;
+}
 
 // XCD-aware block remap: the dispatcher places workgroup b on XCD b % 8, so
 // give each XCD a contiguous run of logical blocks (neighbouring tiles share

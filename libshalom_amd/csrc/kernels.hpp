@@ -25,12 +25,13 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tiles(GemmDesc d, int tiles_n
     const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
     const int kbeg = blockIdx.y * kper;
     const int kend = min(d.K, kbeg + kper);
-    Tile<Cfg> t;
-    t.run(d, m0, n0, kbeg, kend, lds);
+    OneSrc src;
     if (gridDim.y == 1)
-        t.store((typename Cfg::T *)d.C, d.ldc, d.M, d.N, m0, n0);
+        src.job = tile_job<Cfg>(d, m0, n0, kbeg, kend, d.C, d.ldc);
     else
-        t.store(partial + (size_t)blockIdx.y * d.M * d.N, d.N, d.M, d.N, m0, n0);
+        src.job = tile_job<Cfg>(d, m0, n0, kbeg, kend, partial + (size_t)blockIdx.y * d.M * d.N, d.N);
+    Tile<Cfg> t;
+    t.stream(src, 0, 1, lds);
 }
 
 // Many independent GEMMs: one workgroup per (GEMM, tile) work item.
@@ -45,21 +46,25 @@ __global__ __launch_bounds__(kThreads) void k_gemm_batched(const GemmDesc *__res
     const int tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
     const int tm = it.tile / tiles_n, tn = it.tile - tm * tiles_n;
     const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
+    OneSrc src;
+    src.job = tile_job<Cfg>(d, m0, n0, 0, d.K, d.C, d.ldc);
     Tile<Cfg> t;
-    t.run(d, m0, n0, 0, d.K, lds);
-    t.store((typename Cfg::T *)d.C, d.ldc, d.M, d.N, m0, n0);
+    t.stream(src, 0, 1, lds);
 }
 
 // Persistent executor: gridDim.x workgroups (as many as can be resident)
 // stream a list of tile jobs, workgroup b taking jobs b, b + grid, ...
+// Two resident workgroups per CU for the 128 x 128 tiles (the register file
+// then allows 256 per lane), which keeps two job streams' loads in flight.
 template <class Cfg>
 __global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
-                          amdgpu_waves_per_eu(Cfg::WG_PER_CU, 8))) void
+                          amdgpu_waves_per_eu(Cfg::BM * Cfg::BN >= 128 * 128 ? 2 : 1, 8))) void
 k_gemm_persistent(const GemmDesc *__restrict__ jobs, int njobs)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
+    ArraySrc src{jobs, njobs};
     Tile<Cfg> t;
-    t.stream_jobs(jobs, njobs, blockIdx.x, gridDim.x, lds);
+    t.stream(src, blockIdx.x, gridDim.x, lds);
 }
 
 // C = sum over slices of partial[s], in slice order.
