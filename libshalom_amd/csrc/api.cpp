@@ -28,11 +28,16 @@ LaunchTable &launch_table()
     static LaunchTable *t = [] {
         LaunchTable *p = new LaunchTable();
         std::memset(p, 0, sizeof(LaunchTable));
-        register_fp32(*p);
-        register_bf16x3(*p);
-        register_bf16(*p);
-        register_bf16x6(*p);
-        register_fp64(*p);
+        register_fp32_ta0(*p);
+        register_fp32_ta1(*p);
+        register_bf16x3_ta0(*p);
+        register_bf16x3_ta1(*p);
+        register_bf16_ta0(*p);
+        register_bf16_ta1(*p);
+        register_bf16x6_ta0(*p);
+        register_bf16x6_ta1(*p);
+        register_fp64_ta0(*p);
+        register_fp64_ta1(*p);
         return p;
     }();
     return *t;
@@ -42,6 +47,7 @@ LaunchTable &launch_table()
 namespace {
 
 thread_local int g_last_status = LIBSHALOM_SUCCESS;
+constexpr long kMaxLd = 1L << 21;  // leading dimensions (elements) the kernels' 32-bit offsets allow
 std::atomic<int> g_precision{LIBSHALOM_FP32};
 std::atomic<int> g_thread_nums{64}, g_tm{0}, g_tn{0};
 
@@ -144,8 +150,9 @@ int check_dims(int ta, int tb, long M, long N, long K)
     if (M < 0 || N < 0 || K < 0) return LIBSHALOM_INVALID_VALUE;
     const long lim = (1L << 31) - 1;
     if (M > lim || N > lim || K > lim) return LIBSHALOM_NOT_SUPPORTED;
-    if ((double)M * (double)K > 9.0e18 || (double)K * (double)N > 9.0e18 || (double)M * (double)N > 9.0e18)
-        return LIBSHALOM_NOT_SUPPORTED;
+    // the kernels address a tile's rows with 32-bit byte offsets
+    const long ld_lim = kMaxLd;
+    if ((ta ? M : K) >= ld_lim || (tb ? K : N) >= ld_lim || N >= ld_lim) return LIBSHALOM_NOT_SUPPORTED;
     return LIBSHALOM_SUCCESS;
 }
 
@@ -173,11 +180,8 @@ GemmDesc make_desc(int ta, int tb, const void *A, const void *B, void *C, long M
     d.lda = (int)(ta ? M : K);
     d.ldb = (int)(tb ? K : N);
     d.ldc = (int)N;
-    const int vec = 16 / esize;
-    int flags = 0;
-    if (((uintptr_t)A % 16) == 0 && d.lda % vec == 0) flags |= 1;
-    if (((uintptr_t)B % 16) == 0 && d.ldb % vec == 0) flags |= 2;
-    d.flags = flags;
+    (void)esize;
+    d.flags = 0;
     return d;
 }
 
@@ -259,49 +263,46 @@ struct LibShalom_plan_s {
 
 namespace {
 
-// How a batched plan executes its buckets.
+// How a batched plan executes.
 enum BatchMode : int {
     BM_PERSIST = 0,  // one persistent launch per tile bucket, jobs streamed across
-    BM_GRID = 1,     // one workgroup per (GEMM, tile) work item
-    BM_SINGLE = 2,   // every GEMM in the largest square-tile bucket: one persistent launch
+    BM_SINGLE = 1,   // every GEMM in the largest square-tile bucket: one persistent launch
 };
 
 int batch_mode_from_env()
 {
     const char *m = std::getenv("LIBSHALOM_BATCHED_MODE");
-    if (!m) return BM_PERSIST;
-    if (!std::strcmp(m, "grid")) return BM_GRID;
-    if (!std::strcmp(m, "single")) return BM_SINGLE;
+    if (m && !std::strcmp(m, "single")) return BM_SINGLE;
     return BM_PERSIST;
 }
 
 // A tile of a GEMM as a GEMM of its own: pointers offset to the tile, M and N
-// cut to the tile, alignment flags recomputed.
+// cut to the tile.
 GemmDesc tile_job(int ta, int tb, const GemmDesc &g, int m0, int n0, int bm, int bn, int esize)
 {
     GemmDesc j = g;
     const char *A = (const char *)g.A, *B = (const char *)g.B;
     char *C = (char *)g.C;
-    j.A = A + (size_t)esize * (ta ? (size_t)m0 : (size_t)m0 * g.lda);
-    j.B = B + (size_t)esize * (tb ? (size_t)n0 * g.ldb : (size_t)n0);
+    if (g.K > 0) {
+        j.A = A + (size_t)esize * (ta ? (size_t)m0 : (size_t)m0 * g.lda);
+        j.B = B + (size_t)esize * (tb ? (size_t)n0 * g.ldb : (size_t)n0);
+    }
     j.C = C + (size_t)esize * ((size_t)m0 * g.ldc + n0);
     j.M = std::min(bm, g.M - m0);
     j.N = std::min(bn, g.N - n0);
-    const int vec = 16 / esize;
-    j.flags = 0;
-    if (((uintptr_t)j.A % 16) == 0 && j.lda % vec == 0) j.flags |= 1;
-    if (((uintptr_t)j.B % 16) == 0 && j.ldb % vec == 0) j.flags |= 2;
     return j;
 }
 
+// Relative cost of a job for ordering and for apportioning workgroups: the
+// kernels are bound by moving operands, so count elements moved.
+double job_cost(const GemmDesc &d) { return (double)(d.M + d.N) * std::max(d.K, 1) + (double)d.M * d.N; }
+
 struct Bucket {
     int tile;
-    std::vector<GemmDesc> descs;  // GEMMs (grid mode) or tile jobs (persistent modes)
-    std::vector<WorkItem> items;  // grid mode only
-    int grid = 0;                 // persistent launch size
+    std::vector<GemmDesc> jobs;
+    int grid = 0;
+    size_t off = 0;  // byte offset of the bucket's jobs in the plan buffer
 };
-
-double job_cost(const GemmDesc &d) { return (double)d.M * d.N * std::max(d.K, 1); }
 
 int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, const void *const *A,
                const void *const *B, const long *M, const long *N, const long *K, long count)
@@ -334,36 +335,26 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
         const int tile = mode == BM_SINGLE ? single_tile : batched_tile(prec, M[g], N[g], K[g]);
         const GemmDesc d = make_desc(ta, tb, A[g], B[g], C[g], M[g], N[g], K[g], esize);
         Bucket &b = buckets[tile];
-        if (mode == BM_GRID) {
-            b.descs.push_back(d);
-        } else {
-            const TileShape ts = kTileShapes[tile];
-            for (int m0 = 0; m0 < d.M; m0 += ts.bm)
-                for (int n0 = 0; n0 < d.N; n0 += ts.bn) b.descs.push_back(tile_job(ta, tb, d, m0, n0, ts.bm, ts.bn, esize));
-        }
+        const TileShape ts = kTileShapes[tile];
+        for (int m0 = 0; m0 < d.M; m0 += ts.bm)
+            for (int n0 = 0; n0 < d.N; n0 += ts.bn) b.jobs.push_back(tile_job(ta, tb, d, m0, n0, ts.bm, ts.bn, esize));
     }
-    // Longest work first inside a bucket (persistent workgroups take jobs
-    // round-robin, so each gets a similar mix); biggest tiles first across buckets.
-    long total_items = 0;
+    // Per bucket: longest jobs first (workgroups take jobs round-robin, so
+    // each gets a similar mix), as many workgroups as can be resident.
+    long total_jobs = 0;
+    size_t total = 0;
     for (auto &b : buckets) {
-        std::stable_sort(b.descs.begin(), b.descs.end(),
+        const long nj = (long)b.jobs.size();
+        if (nj == 0) continue;
+        if (nj > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
+        std::stable_sort(b.jobs.begin(), b.jobs.end(),
                          [](const GemmDesc &x, const GemmDesc &y) { return job_cost(x) > job_cost(y); });
-        if (mode == BM_GRID) {
-            const TileShape ts = kTileShapes[b.tile];
-            for (int i = 0; i < (int)b.descs.size(); ++i) {
-                const GemmDesc &d = b.descs[i];
-                const int nt = ((d.M + ts.bm - 1) / ts.bm) * ((d.N + ts.bn - 1) / ts.bn);
-                for (int t = 0; t < nt; ++t) b.items.push_back(WorkItem{i, t});
-            }
-            total_items += (long)b.items.size();
-            if ((long)b.items.size() > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
-        } else if (!b.descs.empty()) {
-            total_items += (long)b.descs.size();
-            if ((long)b.descs.size() > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
-            PersistOccupancy occ = launch_table().persist_occ[prec][ta][tb][b.tile];
-            const int per_cu = occ ? std::max(1, occ()) : 1;
-            b.grid = (int)std::min<long>((long)b.descs.size(), (long)per_cu * info.num_cu);
-        }
+        total_jobs += nj;
+        PersistOccupancy occ = launch_table().persist_occ[prec][ta][tb][b.tile];
+        const long slots = (long)std::max(1, occ ? occ() : 1) * info.num_cu;
+        b.grid = (int)std::min(nj, slots);
+        b.off = total;
+        total += ((size_t)nj * sizeof(GemmDesc) + 255) & ~(size_t)255;
     }
 
     LibShalom_plan_s *plan = new (std::nothrow) LibShalom_plan_s();
@@ -372,16 +363,8 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     plan->prec = prec;
     plan->flops = flops;
     plan->bytes = bytes;
-    plan->items = total_items;
+    plan->items = total_jobs;
 
-    size_t total = 0;
-    std::vector<std::pair<size_t, size_t>> offs(NTILE);
-    for (auto &b : buckets) {
-        offs[b.tile].first = total;
-        total += (b.descs.size() * sizeof(GemmDesc) + 255) & ~(size_t)255;
-        offs[b.tile].second = total;
-        total += (b.items.size() * sizeof(WorkItem) + 255) & ~(size_t)255;
-    }
     hipStream_t cs = nullptr;
     auto fail = [&](int code) {
         if (cs) (void)hipStreamDestroy(cs);
@@ -395,12 +378,8 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
             return fail(LIBSHALOM_ALLOC_FAILED);
         }
         std::vector<char> host(total);
-        for (auto &b : buckets) {
-            if (!b.descs.empty())
-                std::memcpy(host.data() + offs[b.tile].first, b.descs.data(), b.descs.size() * sizeof(GemmDesc));
-            if (!b.items.empty())
-                std::memcpy(host.data() + offs[b.tile].second, b.items.data(), b.items.size() * sizeof(WorkItem));
-        }
+        for (auto &b : buckets)
+            if (!b.jobs.empty()) std::memcpy(host.data() + b.off, b.jobs.data(), b.jobs.size() * sizeof(GemmDesc));
         if (hipMemcpy(plan->dev_buf, host.data(), total, hipMemcpyHostToDevice) != hipSuccess)
             return fail(LIBSHALOM_HIP_ERROR);
     }
@@ -411,16 +390,10 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     bool launch_ok = true;
     for (int oi = 0; oi < NTILE; ++oi) {
         const Bucket &b = buckets[order[oi]];
-        if (b.descs.empty()) continue;
-        const GemmDesc *dd = (const GemmDesc *)((char *)plan->dev_buf + offs[b.tile].first);
-        if (mode == BM_GRID) {
-            BatchedLaunch fn = launch_table().batched[prec][ta][tb][b.tile];
-            const WorkItem *di = (const WorkItem *)((char *)plan->dev_buf + offs[b.tile].second);
-            if (!fn || fn(dd, di, (int)b.items.size(), cs) != hipSuccess) launch_ok = false;
-        } else {
-            PersistLaunch fn = launch_table().persist[prec][ta][tb][b.tile];
-            if (!fn || fn(dd, (int)b.descs.size(), b.grid, cs) != hipSuccess) launch_ok = false;
-        }
+        if (b.grid == 0) continue;
+        const GemmDesc *jobs = (const GemmDesc *)((char *)plan->dev_buf + b.off);
+        PersistLaunch fn = launch_table().persist[prec][ta][tb][b.tile];
+        if (!fn || fn(jobs, (int)b.jobs.size(), b.grid, cs) != hipSuccess) launch_ok = false;
     }
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(cs, &g);

@@ -51,13 +51,7 @@ struct GemmDesc {
     void *C;
     int M, N, K;
     int lda, ldb, ldc;
-    int flags;  // bit 0: A 16-byte vector loads allowed, bit 1: B
-};
-
-// Work item of a batched launch: which GEMM and which tile of it.
-struct WorkItem {
-    int gemm;
-    int tile;  // tm * tiles_n + tn
+    int flags;  // reserved (0)
 };
 
 template <int PREC> struct PrecInfo;
@@ -68,6 +62,77 @@ template <> struct PrecInfo<P_BF16X3> { typedef float T; static constexpr int pl
 template <> struct PrecInfo<P_BF16X6> { typedef float T; static constexpr int planes = 3; };
 
 __device__ __forceinline__ int uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
+// Global-memory (address space 1) views of the operand and result pointers.
+// Through a generic pointer the compiler emits flat_load / flat_store, which
+// also count in lgkmcnt, so every wait on an LDS read would drain the global
+// prefetches as well; global_load / global_store keep the two apart.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename T> using gcptr = const __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gcptr<T> to_global(const T *p) { return (gcptr<T>)p; }
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T *p) { return (gptr<T>)p; }
+
+// Raw buffer resources.  Loads and stores go through buffer instructions with
+// a per-lane byte offset; a lane that must not touch memory gets kOOB, an
+// offset past num_records, for which the hardware returns zeros (loads) or
+// drops the write (stores).  So no load or store sits in a divergent branch,
+// the number of memory instructions per step is static, and the compiler can
+// wait for one chunk's loads (vmcnt(n)) while the next chunk's stay in
+// flight.  Offsets are 32-bit: leading dimensions are limited to < 2^21.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kOOB = (int)0x80000000u;
+
+// off if ok, else an offset past num_records (bit 31 set); branch-free.
+__device__ __forceinline__ int oob_unless(bool ok, int off) { return off | (int)((unsigned)(!ok) << 31); }
+
+// nrec: bytes addressable from base.  The range check is per dword (measured
+// on gfx950): a 16-byte load that runs past nrec returns its in-range dwords
+// and zeros for the rest, and 16-byte loads need only 4-byte alignment.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void *base, int nrec = 0x7FFFFFFF)
+{
+    // stride 0, word 3 = gfx9-family data format
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, nrec, 0x00020000);
+}
+
+template <typename T>
+__device__ __forceinline__ T buf_load1(__amdgpu_buffer_rsrc_t r, int off)
+{
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+// 16 bytes: four floats or two doubles.  The whole vector is bit-cast
+// before its elements are taken: extracting lanes of the u32x4 result one by
+// one made the compiler shrink the load to a single dword.
+template <typename T>
+__device__ __forceinline__ void buf_load16(__amdgpu_buffer_rsrc_t r, int off, T *v)
+{
+    if constexpr (sizeof(T) == 4) {
+        const f32x4 x = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+        v[0] = x[0];
+        v[1] = x[1];
+        v[2] = x[2];
+        v[3] = x[3];
+    } else {
+        const f64x2 x = __builtin_bit_cast(f64x2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+        v[0] = x[0];
+        v[1] = x[1];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t r, int off, T x)
+{
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, off, 0, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, off, 0, 0);
+}
 
 // Split x into bf16 pieces, x ~= p0 + p1 + p2, each round-to-nearest-even of
 // the remaining residual (the residual subtraction is exact in FP32).
@@ -154,9 +219,14 @@ __device__ __forceinline__ int kpos(int k) { return (k & 3) * NSLOT + (k >> 2); 
 
 // ---------------------------------------------------------------------------
 // Global -> register staging of one operand panel chunk (ROWS x BK).
-// KCONTIG: storage is [row][k] (stride ld); otherwise [k][row].  LANEROW
-// (BF16 modes, row-contiguous operand): lane-per-row 4-byte loads, each thread
-// holding VEC consecutive k of one row.
+// KCONTIG: storage is [row][k] (stride ld); otherwise [k][row].  Each thread
+// holds groups of VEC elements consecutive along the contiguous dimension,
+// loaded by one 16-byte buffer load (any 4-byte alignment); rows past the
+// tile take an out-of-range offset and elements past the K / row tail are
+// zeroed by a select.  The buffer's num_records is the job's exact extent,
+// so a group running past the end of the matrix reads nothing beyond it.
+// LANEROW (BF16 modes, row-contiguous operand): lane-per-row 4-byte loads,
+// each thread holding VEC consecutive k of one row.
 // ---------------------------------------------------------------------------
 template <typename T, int NP, int ROWS, int BK, bool KCONTIG>
 struct Stage {
@@ -165,6 +235,7 @@ struct Stage {
     static constexpr bool LANEROW = NP > 0 && !KCONTIG;
     static constexpr int NVEC = ROWS * BK / VEC;
     static constexpr int PER = (NVEC + kThreads - 1) / kThreads;
+    static constexpr int ES = sizeof(T);
     T v[PER][VEC];
 
     // First element (row, k) of the thread's group of VEC elements.
@@ -182,63 +253,36 @@ struct Stage {
         }
     }
 
-    __device__ __forceinline__ void load(const T *__restrict__ base, int ld, int R, int kend,
-                                         int row0, int k0, bool vec_ok, int tid)
+    // Load the chunk whose first k is k0 (rows [0, R), k < kend of the job).
+    // live == false issues the same loads with nothing in range.
+    __device__ __forceinline__ void load(const T *__restrict__ base, int ld, int R, int kend, int k0, bool live,
+                                         int tid)
     {
+        const int kn = kend - k0;  // valid k in this chunk
+        // bytes from the chunk base to one past the job's last element
+        const int nrec = (live && kn > 0 && R > 0) ? (KCONTIG ? ((R - 1) * ld + min(kn, BK)) * ES
+                                                              : ((min(kn, BK) - 1) * ld + R) * ES)
+                                                   : 0;
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(KCONTIG ? base + k0 : base + (size_t)k0 * ld, nrec);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int idx = tid + i * kThreads;
             int row, k;
             coords(idx, row, k);
-            const int gr = row0 + row, gk = k0 + k;
-#pragma unroll
-            for (int e = 0; e < VEC; ++e) v[i][e] = T(0);
-            const bool active = (NVEC % kThreads == 0) || idx < NVEC;
+            const bool act = (NVEC % kThreads == 0) | (idx < NVEC);
             if constexpr (KCONTIG) {
-                if (active && gr < R) {
-                    const T *p = base + (size_t)gr * ld + gk;
-                    if (vec_ok && gk + VEC <= kend) {
-                        if constexpr (sizeof(T) == 4) {
-                            f32x4 t = *(const f32x4 *)p;
+                buf_load16<T>(rs, oob_unless(act & (row < R) & (k < kn), (row * ld + k) * ES), v[i]);
 #pragma unroll
-                            for (int e = 0; e < VEC; ++e) v[i][e] = t[e];
-                        } else {
-                            f64x2 t = *(const f64x2 *)p;
-#pragma unroll
-                            for (int e = 0; e < VEC; ++e) v[i][e] = t[e];
-                        }
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < VEC; ++e)
-                            if (gk + e < kend) v[i][e] = p[e];
-                    }
-                }
+                for (int e = 1; e < VEC; ++e) v[i][e] = (k + e < kn) ? v[i][e] : T(0);
             } else if constexpr (LANEROW) {
-                if (active && gr < R) {
-                    const T *p = base + (size_t)gk * ld + gr;
+                const bool rok = act & (row < R);
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e)
-                        if (gk + e < kend) v[i][e] = p[(size_t)e * ld];
-                }
+                for (int e = 0; e < VEC; ++e)
+                    v[i][e] = buf_load1<T>(rs, oob_unless(rok & (k + e < kn), ((k + e) * ld + row) * ES));
             } else {
-                if (active && gk < kend) {
-                    const T *p = base + (size_t)gk * ld + gr;
-                    if (vec_ok && gr + VEC <= R) {
-                        if constexpr (sizeof(T) == 4) {
-                            f32x4 t = *(const f32x4 *)p;
+                buf_load16<T>(rs, oob_unless(act & (k < kn) & (row < R), (k * ld + row) * ES), v[i]);
 #pragma unroll
-                            for (int e = 0; e < VEC; ++e) v[i][e] = t[e];
-                        } else {
-                            f64x2 t = *(const f64x2 *)p;
-#pragma unroll
-                            for (int e = 0; e < VEC; ++e) v[i][e] = t[e];
-                        }
-                    } else {
-#pragma unroll
-                        for (int e = 0; e < VEC; ++e)
-                            if (gr + e < R) v[i][e] = p[e];
-                    }
-                }
+                for (int e = 1; e < VEC; ++e) v[i][e] = (row + e < R) ? v[i][e] : T(0);
             }
         }
     }
@@ -440,12 +484,13 @@ struct Tile {
         zero();
     }
 
-    // Global -> registers: the K chunk at k0 of the tile at (m0, n0).
-    __device__ __forceinline__ static void load_chunk(StageA &sa, StageB &sb, const GemmDesc &d, int m0, int n0,
-                                                      int k0, int kend, int tid)
+    // Global -> registers: the K chunk at k0 of job d (live == false: a
+    // placeholder load that returns zeros, keeping instruction counts static).
+    __device__ __forceinline__ static void load_chunk(StageA &sa, StageB &sb, const GemmDesc &d, int k0, bool live,
+                                                      int tid)
     {
-        sa.load((const T *)d.A, d.lda, d.M, kend, m0, k0, d.flags & 1, tid);
-        sb.load((const T *)d.B, d.ldb, d.N, kend, n0, k0, d.flags & 2, tid);
+        sa.load((const T *)d.A, d.lda, d.M, d.K, k0, live, tid);
+        sb.load((const T *)d.B, d.ldb, d.N, d.K, k0, live, tid);
     }
 
     // Registers -> LDS in the packed MFMA operand layout (transpose / split).
@@ -499,9 +544,9 @@ struct Tile {
                                          StageB &b1, char *lds, int bi, const Src &src, int stride, int tid,
                                          int lane, int wave)
     {
-        if (n2.job >= 0) load_chunk(a0, b0, n2.d, 0, 0, n2.c * BK, n2.d.K, tid);
+        load_chunk(a0, b0, n2.d, n2.c * BK, n2.job >= 0, tid);
         compute(lds_buf(lds, bi), lane, ksteps(cur.d.K, cur.c * BK));
-        if (cur.c + 1 == cur.nch) store((T *)cur.d.C, cur.d.ldc, cur.d.M, cur.d.N, 0, 0);
+        if (cur.c + 1 == cur.nch) store((T *)cur.d.C, cur.d.ldc, cur.d.M, cur.d.N);
         if (n1.job < 0) return false;
         if (n1.c == 0) begin(n1.d.M, n1.d.N, 0, 0, wave);
         if constexpr (Cfg::NBUF == 1) __syncthreads();
@@ -537,8 +582,8 @@ struct Tile {
         advance(n2, src, stride);
         StageA a0, a1;
         StageB b0, b1;
-        load_chunk(a0, b0, cur.d, 0, 0, 0, cur.d.K, tid);
-        if (n1.job >= 0) load_chunk(a1, b1, n1.d, 0, 0, n1.c * BK, n1.d.K, tid);
+        load_chunk(a0, b0, cur.d, 0, true, tid);
+        load_chunk(a1, b1, n1.d, n1.c * BK, n1.job >= 0, tid);
         begin(cur.d.M, cur.d.N, 0, 0, wave);
         chunk_to_lds(a0, b0, lds_buf(lds, 0), tid);
         __syncthreads();
@@ -560,11 +605,13 @@ struct Tile {
             return 4 * q + i;
     }
 
-    // Store the accumulators to a row-major OutT matrix (C or a split-K
-    // partial) with leading dimension ldo; rows >= M / cols >= N are masked.
+    // Store the accumulators to a row-major OutT matrix (a C tile or a split-K
+    // partial) with leading dimension ldo; rows >= M / cols >= N are dropped
+    // by the buffer range check.
     template <typename OutT>
-    __device__ __forceinline__ void store(OutT *__restrict__ out, int ldo, int M, int N, int m0, int n0) const
+    __device__ __forceinline__ void store(OutT *__restrict__ out, int ldo, int M, int N) const
     {
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(out);
         const int lane = threadIdx.x & (kWave - 1);
         const int r = lane & 15, q = lane >> 4;
 #pragma unroll
@@ -572,14 +619,13 @@ struct Tile {
 #pragma unroll
             for (int b = 0; b < WN; ++b)
                 if (a < wm && b < wn) {
-                    const int col = n0 + (wc0 + b) * 16 + r;
-                    const int row0 = m0 + (wr0 + a) * 16;
-                    if (col < N) {
+                    const int col = (wc0 + b) * 16 + r;
+                    const int row0 = (wr0 + a) * 16;
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const int row = row0 + acc_row(q, i);
-                            if (row < M) out[(size_t)row * ldo + col] = (OutT)acc[a][b][i];
-                        }
+                    for (int i = 0; i < 4; ++i) {
+                        const int row = row0 + acc_row(q, i);
+                        const int off = oob_unless((row < M) & (col < N), (row * ldo + col) * (int)sizeof(OutT));
+                        buf_store1<OutT>(rs, off, (OutT)acc[a][b][i]);
                     }
                 }
     }
@@ -590,7 +636,18 @@ struct ArraySrc {
     const GemmDesc *__restrict__ jobs;
     int n;
     __device__ __forceinline__ int count() const { return n; }
-    __device__ __forceinline__ GemmDesc get(int j) const { return jobs[j]; }
+    // The descriptor is read through the constant address space with a
+    // uniform index, so it arrives by scalar loads (lgkmcnt), not vector
+    // loads that would count against the operand prefetches.
+    __device__ __forceinline__ GemmDesc get(int j) const
+    {
+#if __HIP_DEVICE_COMPILE__
+        typedef const __attribute__((address_space(4))) GemmDesc *cdesc;
+        return ((cdesc)jobs)[uniform(j)];
+#else
+        return jobs[j];  // host pass only parses device functions
+#endif
+    }
 };
 
 // Job source of exactly one job held in registers.
@@ -619,10 +676,7 @@ __device__ __forceinline__ GemmDesc tile_job(const GemmDesc &g, int m0, int n0, 
     j.lda = g.lda;
     j.ldb = g.ldb;
     j.ldc = ldo;
-    int f = 0;
-    if ((((uintptr_t)j.A) & 15) == 0 && g.lda % Cfg::VEC == 0) f |= 1;
-    if ((((uintptr_t)j.B) & 15) == 0 && g.ldb % Cfg::VEC == 0) f |= 2;
-    j.flags = f;
+    j.flags = g.flags;
     return j;
 }
 

@@ -1,0 +1,14 @@
+// kern_bf16x3_ta0.hip -- tile-GEMM instantiations for the bf16x3 mode with
+// transa = 0 (one translation unit per precision and transa so the kernels
+// compile in parallel).
+#include "kernels.hpp"
+
+namespace ls {
+
+void register_bf16x3_ta0(LaunchTable &t)
+{
+    register_trans<P_BF16X3, 0, 0>(t);
+    register_trans<P_BF16X3, 0, 1>(t);
+}
+
+}  // namespace ls

@@ -34,28 +34,10 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tiles(GemmDesc d, int tiles_n
     t.stream(src, 0, 1, lds);
 }
 
-// Many independent GEMMs: one workgroup per (GEMM, tile) work item.
-template <class Cfg>
-__global__ __launch_bounds__(kThreads) void k_gemm_batched(const GemmDesc *__restrict__ descs,
-                                                          const WorkItem *__restrict__ items, int nitems)
-{
-    __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
-    const int b = xcd_remap(blockIdx.x, nitems);
-    const WorkItem it = items[b];
-    const GemmDesc d = descs[it.gemm];
-    const int tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
-    const int tm = it.tile / tiles_n, tn = it.tile - tm * tiles_n;
-    const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
-    OneSrc src;
-    src.job = tile_job<Cfg>(d, m0, n0, 0, d.K, d.C, d.ldc);
-    Tile<Cfg> t;
-    t.stream(src, 0, 1, lds);
-}
-
-// Persistent executor: gridDim.x workgroups (as many as can be resident)
-// stream a list of tile jobs, workgroup b taking jobs b, b + grid, ...
-// Two resident workgroups per CU for the 128 x 128 tiles (the register file
-// then allows 256 per lane), which keeps two job streams' loads in flight.
+// Persistent executor: as many workgroups as can be resident stream the job
+// list, workgroup b taking jobs b, b + grid, ...  Two resident workgroups per CU for the
+// 128 x 128 tiles (the register file then allows 256 per lane), which keeps
+// two job streams' loads in flight.
 template <class Cfg>
 __global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
                           amdgpu_waves_per_eu(Cfg::BM * Cfg::BN >= 128 * 128 ? 2 : 1, 8))) void
@@ -83,13 +65,11 @@ __global__ __launch_bounds__(256) void k_splitk_reduce(const T *__restrict__ par
 // Launcher table (host side).
 // ---------------------------------------------------------------------------
 typedef hipError_t (*TilesLaunch)(const GemmDesc &d, int ksplit, int kper, void *partial, hipStream_t s);
-typedef hipError_t (*BatchedLaunch)(const GemmDesc *descs, const WorkItem *items, int nitems, hipStream_t s);
 typedef hipError_t (*PersistLaunch)(const GemmDesc *jobs, int njobs, int grid, hipStream_t s);
 typedef int (*PersistOccupancy)();  // resident workgroups per CU, 0 on error
 
 struct LaunchTable {
     TilesLaunch tiles[NPREC][2][2][NTILE];
-    BatchedLaunch batched[NPREC][2][2][NTILE];
     PersistLaunch persist[NPREC][2][2][NTILE];
     PersistOccupancy persist_occ[NPREC][2][2][NTILE];
 };
@@ -103,13 +83,6 @@ hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, 
     dim3 grid(ntiles, ksplit);
     hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(kThreads), 0, s, d, tiles_n, ntiles, kper,
                        (typename Cfg::T *)partial);
-    return hipGetLastError();
-}
-
-template <class Cfg>
-hipError_t launch_batched(const GemmDesc *descs, const WorkItem *items, int nitems, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_gemm_batched<Cfg>, dim3(nitems), dim3(kThreads), 0, s, descs, items, nitems);
     return hipGetLastError();
 }
 
@@ -136,7 +109,6 @@ void register_one(LaunchTable &t, int tile)
 {
     typedef TileCfg<PREC, TA, TB, BM, BN> Cfg;
     t.tiles[PREC][TA][TB][tile] = &launch_tiles<Cfg>;
-    t.batched[PREC][TA][TB][tile] = &launch_batched<Cfg>;
     // the batched buckets use the square tiles
     if (BM == BN) {
         t.persist[PREC][TA][TB][tile] = &launch_persistent<Cfg>;
@@ -155,11 +127,16 @@ void register_trans(LaunchTable &t)
     register_one<PREC, TA, TB, 32, 128>(t, T32x128);
 }
 
-// Defined in one translation unit per precision.
-void register_fp32(LaunchTable &t);
-void register_bf16x3(LaunchTable &t);
-void register_bf16(LaunchTable &t);
-void register_bf16x6(LaunchTable &t);
-void register_fp64(LaunchTable &t);
+// Defined in one translation unit per precision and transa.
+void register_fp32_ta0(LaunchTable &t);
+void register_fp32_ta1(LaunchTable &t);
+void register_bf16x3_ta0(LaunchTable &t);
+void register_bf16x3_ta1(LaunchTable &t);
+void register_bf16_ta0(LaunchTable &t);
+void register_bf16_ta1(LaunchTable &t);
+void register_bf16x6_ta0(LaunchTable &t);
+void register_bf16x6_ta1(LaunchTable &t);
+void register_fp64_ta0(LaunchTable &t);
+void register_fp64_ta1(LaunchTable &t);
 
 }  // namespace ls
