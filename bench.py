@@ -37,7 +37,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--modes", default="bf16x3,fp32,bf16x6,bf16",
-                    help="precision modes to time (the first listed of bf16x3 is the headline)")
+                    help="precision modes to time besides the headline bf16x3")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="issue one device-to-device copy of this many MiB first (untimed), a known-size "
+                         "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the batched path "
@@ -51,6 +54,14 @@ def main():
     from libshalom_amd.workloads import uniform_shapes
 
     dev = torch.device("cuda:0")
+    if args.calib_copy_mib > 0:
+        n = args.calib_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     shapes = uniform_shapes(args.count, 1, 128, seed=0)
     Ms = np.array([s[0] for s in shapes], dtype=np.int64)
     Ns = np.array([s[1] for s in shapes], dtype=np.int64)
@@ -127,6 +138,33 @@ def main():
         if name == "bf16x3":
             headline = r
 
+    # Dominant kernel: the 128 x 128-tile persistent bucket (GEMMs with
+    # max(M, N) > 64).  A plan holding only those GEMMs is one kernel launch;
+    # HIP events on its stream give that kernel's average duration.
+    big = [g for g in range(args.count) if ls.query_batched_tile(ls.BF16X3, int(Ms[g]), int(Ns[g]), int(Ks[g]))
+           == (128, 128)]
+    bidx = np.array(big, dtype=np.int64)
+    kplan = ls.BatchedPlan(0, 0, [Cs[g] for g in big], [As[g] for g in big], [Bs[g] for g in big],
+                           Ms[bidx], Ns[bidx], Ks[bidx], precision=ls.BF16X3)
+    kbytes = float(np.sum(4.0 * (Ms[bidx] * Ks[bidx] + Ks[bidx] * Ns[bidx] + Ms[bidx] * Ns[bidx])))
+    kflops = float(np.sum(2.0 * Ms[bidx] * Ns[bidx] * Ks[bidx]))
+    for _ in range(args.warmup):
+        kplan.execute(stream)
+    k0 = torch.cuda.Event(enable_timing=True)
+    k1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    k0.record(stream)
+    for _ in range(args.steps):
+        kplan.execute(stream)
+    k1.record(stream)
+    torch.cuda.synchronize()
+    kms = k0.elapsed_time(k1) / args.steps
+    kplan.close()
+    kgbs = kbytes / (kms * 1e-3) / 1e9
+    log(f"dominant kernel (128x128 bucket, {len(big)} GEMMs): {kms:.3f} ms/launch, {kgbs:.0f} GB/s, "
+        f"{kflops / (kms * 1e-3) / 1e12:.1f} TFLOP/s")
+    traffic = recorded_traffic()
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(shapes, args.cpu_seconds)
@@ -153,12 +191,29 @@ def main():
                                    if "fp32" in results else None),
         "modes": {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
                   for k, v in results.items()},
-        "roofline": {"bound": "hbm", "achieved": round(h["hbm_gbs"], 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(h["hbm_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
-                     "scope": "whole step (all bucket kernels of the graph)"},
+        "roofline": {"bound": "hbm", "achieved": round(kgbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(kgbs / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "kernel": "k_gemm_persistent<bf16x3, NN, 128x128> (the GEMMs with max(M,N) > 64)",
+                     "kernel_ms": round(kms, 4), "kernel_algorithmic_bytes": kbytes,
+                     "kernel_tflops": round(kflops / (kms * 1e-3) / 1e12, 2)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def recorded_traffic():
+    """HBM bytes per launch of the dominant kernel as measured by rocprofv3
+    (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy) and recorded in
+    profiles/; None when no such record exists."""
+    path = os.path.join(ROOT, "profiles", "batched_sgemm_bf16x3_128_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline(shapes, seconds):

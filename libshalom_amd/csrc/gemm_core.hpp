@@ -254,14 +254,17 @@ struct Stage {
     }
 
     // Load the chunk whose first k is k0 (rows [0, R), k < kend of the job).
-    // live == false issues the same loads with nothing in range.
+    // live == false issues the same loads with nothing in range.  The buffer
+    // covers exactly the job's part of the chunk (through row R-1 and k
+    // kc-1), and ld >= R, ld >= kc, so most out-of-tile elements fall past
+    // num_records on their own: rows >= R of a k-contiguous panel, k >= kc
+    // of a row-contiguous one.  Only what is in range but outside the tile
+    // (k >= kc inside a row, rows >= R inside a k row) is masked.
     __device__ __forceinline__ void load(const T *__restrict__ base, int ld, int R, int kend, int k0, bool live,
                                          int tid)
     {
-        const int kn = kend - k0;  // valid k in this chunk
-        // bytes from the chunk base to one past the job's last element
-        const int nrec = (live && kn > 0 && R > 0) ? (KCONTIG ? ((R - 1) * ld + min(kn, BK)) * ES
-                                                              : ((min(kn, BK) - 1) * ld + R) * ES)
+        const int kc = min(kend - k0, BK);  // valid k in this chunk
+        const int nrec = (live && kc > 0 && R > 0) ? (KCONTIG ? ((R - 1) * ld + kc) * ES : ((kc - 1) * ld + R) * ES)
                                                    : 0;
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(KCONTIG ? base + k0 : base + (size_t)k0 * ld, nrec);
 #pragma unroll
@@ -271,18 +274,18 @@ struct Stage {
             coords(idx, row, k);
             const bool act = (NVEC % kThreads == 0) | (idx < NVEC);
             if constexpr (KCONTIG) {
-                buf_load16<T>(rs, oob_unless(act & (row < R) & (k < kn), (row * ld + k) * ES), v[i]);
+                buf_load16<T>(rs, oob_unless(act & (k < kc), (row * ld + k) * ES), v[i]);
 #pragma unroll
-                for (int e = 1; e < VEC; ++e) v[i][e] = (k + e < kn) ? v[i][e] : T(0);
+                for (int e = 1; e < VEC; ++e) v[i][e] = (k + e < kc) ? v[i][e] : T(0);
             } else if constexpr (LANEROW) {
-                const bool rok = act & (row < R);
+                const int hi = (act & (row < R)) ? 0 : kOOB;
+                const int off = (k * ld + row) * ES;
 #pragma unroll
-                for (int e = 0; e < VEC; ++e)
-                    v[i][e] = buf_load1<T>(rs, oob_unless(rok & (k + e < kn), ((k + e) * ld + row) * ES));
+                for (int e = 0; e < VEC; ++e) v[i][e] = buf_load1<T>(rs, (off + e * ld * ES) | hi);
             } else {
-                buf_load16<T>(rs, oob_unless(act & (k < kn) & (row < R), (k * ld + row) * ES), v[i]);
+                buf_load16<T>(rs, oob_unless(act, (k * ld + row) * ES), v[i]);
 #pragma unroll
-                for (int e = 1; e < VEC; ++e) v[i][e] = (row + e < R) ? v[i][e] : T(0);
+                for (int e = 0; e < VEC; ++e) v[i][e] = (row + e < R) ? v[i][e] : T(0);
             }
         }
     }
@@ -606,27 +609,27 @@ struct Tile {
     }
 
     // Store the accumulators to a row-major OutT matrix (a C tile or a split-K
-    // partial) with leading dimension ldo; rows >= M / cols >= N are dropped
-    // by the buffer range check.
+    // partial) with leading dimension ldo.  The buffer ends right after
+    // element (M-1, N-1), so rows >= M fall past it; columns >= N (in range,
+    // they belong to the next tile) take an out-of-range offset.
     template <typename OutT>
     __device__ __forceinline__ void store(OutT *__restrict__ out, int ldo, int M, int N) const
     {
-        const __amdgpu_buffer_rsrc_t rs = make_rsrc(out);
+        constexpr int ES_OUT = sizeof(OutT);
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(out, ((M - 1) * ldo + N) * ES_OUT);
         const int lane = threadIdx.x & (kWave - 1);
         const int r = lane & 15, q = lane >> 4;
+        const int rstep = (acc_row(0, 1) - acc_row(0, 0)) * ldo * ES_OUT;  // offset between a lane's rows
 #pragma unroll
         for (int a = 0; a < WM; ++a)
 #pragma unroll
             for (int b = 0; b < WN; ++b)
                 if (a < wm && b < wn) {
                     const int col = (wc0 + b) * 16 + r;
-                    const int row0 = (wr0 + a) * 16;
+                    const int row = (wr0 + a) * 16 + acc_row(q, 0);
+                    const int off = (row * ldo + col) * ES_OUT | (col < N ? 0 : kOOB);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const int row = row0 + acc_row(q, i);
-                        const int off = oob_unless((row < M) & (col < N), (row * ldo + col) * (int)sizeof(OutT));
-                        buf_store1<OutT>(rs, off, (OutT)acc[a][b][i]);
-                    }
+                    for (int i = 0; i < 4; ++i) buf_store1<OutT>(rs, off + i * rstep, (OutT)acc[a][b][i]);
                 }
     }
 };
