@@ -184,7 +184,7 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16x3 (fp32 in/out, bf16 MFMA, fp32 accumulate)",
         "data": "synthetic: U[-1,1) fp32 entries, M,N,K~U{1..128} splitmix64(0)",
-        "config": {"workload": "batched SGEMM, 50k GEMMs, M,N,K~U{1..128}, NN, row-major (config 4b)",
+        "config": {"workload": "batched SGEMM, 50k GEMMs, M,N,K~U{1..128}, NN, row-major",
                    "count": args.count, "gflop_per_step": round(flops / 1e9, 3),
                    "gb_per_step": round(nbytes / 1e9, 4)},
         "speedup_bf16x3_vs_fp32": (round(results["bf16x3"]["gflops"] / results["fp32"]["gflops"], 3)

@@ -19,7 +19,8 @@ U32 = 2.0 ** -24
 U64 = 2.0 ** -53
 # Extra relative error per product |a||b| of each float mode (from the split:
 # bf16 keeps 8 significand bits, so a two-piece split leaves ~2^-16 and a
-# three-piece split ~2^-24 of each operand; see DESIGN.md "Precision modes").
+# three-piece split ~2^-24 of each operand; the dropped operand residues and
+# the dropped small cross products add up to the constants below).
 PREC_EXTRA = {ls.FP32: 0.0, ls.BF16X6: 4 * 2.0 ** -24, ls.BF16X3: 3 * 2.0 ** -16, ls.BF16: 2 * 2.0 ** -8}
 TRANS = [(0, 0), (0, 1), (1, 0), (1, 1)]
 
