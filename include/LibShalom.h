@@ -128,10 +128,22 @@ int LibShalom_dgemm_batched(int transa, int transb, double *const *C, const doub
 /* Backward of C = op(A) op(B) given dC (M x N): writes dA with the storage
  * shape of A and dB with the storage shape of B,
  *   op(A)' = dC op(B)^T,   op(B)' = op(A)^T dC,
- * both GEMMs in one launch (no reference counterpart). */
+ * both products in ONE kernel launch (no reference counterpart: the
+ * reference has no differentiation).  Asynchronous on `stream`. */
 int LibShalom_sgemm_backward(int transa, int transb, const float *dC, const float *A,
                              const float *B, float *dA, float *dB, long M, long N, long K,
                              int precision, LibShalom_stream_t stream);
+
+/* The same for a strided batch: image i uses A + i*strideA, B + i*strideB,
+ * dC + i*strideC and writes dA + i*strideA, dB + i*strideB.  A stride of 0
+ * for A (or B) means the operand is shared by the batch -- the weights of an
+ * im2col convolution -- and its gradient is the sum over the images, written
+ * once.  One launch; split reductions are summed inside it in a fixed order,
+ * so results do not depend on scheduling.  No reference counterpart. */
+int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, const float *A,
+                                     const float *B, float *dA, float *dB, long M, long N, long K,
+                                     long batch, long strideA, long strideB, long strideC,
+                                     int precision, LibShalom_stream_t stream);
 
 /* Planner introspection (host only, no device needed): the workgroup tile
  * (tile_m x tile_n of C), K-split count and K slice length the library would

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/LibShalom.h"
+#include "backward.hpp"
 #include "kernels.hpp"
 #include "planner.hpp"
 
@@ -38,6 +39,19 @@ LaunchTable &launch_table()
         register_bf16x6_ta1(*p);
         register_fp64_ta0(*p);
         register_fp64_ta1(*p);
+        return p;
+    }();
+    return *t;
+}
+BwdTable &bwd_table()
+{
+    static BwdTable *t = [] {
+        BwdTable *p = new BwdTable();
+        std::memset(p, 0, sizeof(BwdTable));
+        register_backward_fp32(*p);
+        register_backward_bf16x3(*p);
+        register_backward_bf16(*p);
+        register_backward_bf16x6(*p);
         return p;
     }();
     return *t;
@@ -236,6 +250,176 @@ int valid_precision(int p)
 
 // LIBSHALOM_* precision flag -> kernel precision id (they coincide).
 int kprec(int p) { return p; }
+
+// ---------------------------------------------------------------------------
+// Fused backward: planning of the dA (X) and dB (Y) problems.
+// ---------------------------------------------------------------------------
+// Per (device, stream) memory for the split-reduction fixup: arrival
+// counters (zero between launches; every launch leaves them zero) and the
+// partial-tile slots.
+struct FixupSpace {
+    char *ptr = nullptr;
+    size_t size = 0;
+};
+std::mutex g_fx_mu;
+std::map<std::pair<int, hipStream_t>, FixupSpace> g_fx;
+constexpr size_t kCounterBytes = 1 << 20;  // 256 K tile counters
+
+__global__ void k_zero_ints(int *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+int fixup_space(int dev, hipStream_t s, size_t slot_bytes, int **counters, float **slots)
+{
+    if (const char *f = std::getenv("LIBSHALOM_FAIL_WORKSPACE"))
+        if (f[0] == '1') return LIBSHALOM_ALLOC_FAILED;  // failure injection for tests
+    std::lock_guard<std::mutex> lk(g_fx_mu);
+    FixupSpace &w = g_fx[{dev, s}];
+    const size_t need = kCounterBytes + slot_bytes;
+    if (w.size < need) {
+        if (w.ptr) {
+            if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+            (void)hipFree(w.ptr);
+            w.ptr = nullptr;
+            w.size = 0;
+        }
+        const size_t sz = std::max(need, (size_t)8 << 20);
+        if (hipMalloc((void **)&w.ptr, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            w.ptr = nullptr;
+            return LIBSHALOM_ALLOC_FAILED;
+        }
+        w.size = sz;
+        // a fill kernel rather than a memset, so that a captured call records a kernel node
+        hipLaunchKernelGGL(k_zero_ints, dim3(256), dim3(256), 0, s, (int *)w.ptr, (int)(kCounterBytes / sizeof(int)));
+        if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    }
+    *counters = (int *)w.ptr;
+    *slots = (float *)(w.ptr + kCounterBytes);
+    return LIBSHALOM_SUCCESS;
+}
+
+long ceil_div(long a, long b) { return (a + b - 1) / b; }
+
+// One gradient as a problem: output R x Cc (leading dimension ldo), left
+// operand R x red, right operand red x Cc, per-image strides; sO == 0 with a
+// batch means one output summed over the images.
+void bwd_side(BwdProblem &P, const void *L, long long sL, int lda, const void *Rt, long long sR, int ldb, void *O,
+              long long sO, int ldo, long R, long Cc, long red, int nimg)
+{
+    std::memset(&P, 0, sizeof(P));
+    P.L = L;
+    P.Rt = Rt;
+    P.O = O;
+    P.sL = sL;
+    P.sR = sR;
+    P.sO = nimg > 1 ? sO : 0;
+    P.lda = lda;
+    P.ldb = ldb;
+    P.ldo = ldo;
+    P.R = (int)R;
+    P.Cc = (int)Cc;
+    P.red = (int)red;
+    P.nimg = nimg;
+}
+
+// Cut the reductions of both problems into slices of about `len` (a
+// multiple of the K chunk) and count units.
+void bwd_slice(BwdProblem &P, int T, long len)
+{
+    const bool shared = P.sO == 0;
+    P.tiles_c = (int)ceil_div(P.Cc, T);
+    P.ntiles = (int)ceil_div(P.R, T) * P.tiles_c;
+    const long outs = shared ? 1 : P.nimg;
+    const long total = shared ? (long)P.nimg * P.red : P.red;
+    if (P.ntiles == 0) {
+        P.units = 0;
+        P.nslices = 1;
+        return;
+    }
+    if (total <= len || P.red == 0) {
+        P.nslices = 1;
+        P.spi = shared ? 0 : 1;
+        P.gimg = shared ? P.nimg : 1;
+        P.slen = P.red;
+    } else if (!shared || P.red >= len) {
+        long spi = ceil_div(P.red, len);
+        const long slen = ceil_div(ceil_div(P.red, spi), 32) * 32;
+        spi = ceil_div(P.red, slen);
+        P.spi = (int)spi;
+        P.slen = (int)slen;
+        P.gimg = 1;
+        P.nslices = (int)(shared ? P.nimg * spi : spi);
+    } else {
+        const long g = std::max(1L, len / std::max(1, P.red));
+        P.spi = 0;
+        P.gimg = (int)g;
+        P.slen = P.red;
+        P.nslices = (int)ceil_div(P.nimg, g);
+    }
+    P.units = (int)(outs * P.ntiles * P.nslices);
+}
+
+int backward_run(int prec, int ta, int tb, const float *dC, const float *A, const float *B, float *dA, float *dB,
+                 long M, long N, long K, int nimg, long sA, long sB, long sC, hipStream_t s)
+{
+    int dev;
+    DeviceInfo info;
+    int st = current_device(&dev, &info);
+    if (st) return st;
+    BwdProblem X, Y;
+    // dA (stored like A): dC op(B)^T, or op(B) dC^T when A is stored K x M
+    if (!ta)
+        bwd_side(X, dC, sC, (int)N, B, sB, (int)(tb ? K : N), dA, sA, (int)K, M, K, N, nimg);
+    else
+        bwd_side(X, B, sB, (int)(tb ? K : N), dC, sC, (int)N, dA, sA, (int)M, K, M, N, nimg);
+    // dB (stored like B): op(A)^T dC, or dC^T op(A) when B is stored N x K
+    if (!tb)
+        bwd_side(Y, A, sA, (int)(ta ? M : K), dC, sC, (int)N, dB, sB, (int)N, K, N, M, nimg);
+    else
+        bwd_side(Y, dC, sC, (int)N, A, sA, (int)(ta ? M : K), dB, sB, (int)K, N, K, M, nimg);
+    // 128 x 128 tiles when every output dimension fills them, else 64 x 64
+    const int big = std::min(std::min(X.R, X.Cc), std::min(Y.R, Y.Cc)) >= 128;
+    const int T = big ? 128 : 64;
+    // slice length: about four units per compute unit, at least 8 K chunks
+    // per slice, and at most 256 MiB of partial tiles
+    auto work = [&](const BwdProblem &P) {
+        const long outs = (P.sO == 0) ? 1 : P.nimg;
+        const long total = (P.sO == 0) ? (long)P.nimg * P.red : P.red;
+        return (double)outs * ceil_div(P.R, T) * ceil_div(P.Cc, T) * std::max(total, 1L);
+    };
+    const double W = work(X) + work(Y);
+    long len = std::max(256L, (long)std::ceil(W / (4.0 * info.num_cu)));
+    len = ceil_div(len, 32) * 32;
+    size_t slot_bytes = 0;
+    for (int it = 0; it < 32; ++it) {
+        bwd_slice(X, T, len);
+        bwd_slice(Y, T, len);
+        slot_bytes = 0;
+        if (X.nslices > 1) slot_bytes += (size_t)X.units * T * T * sizeof(float);
+        if (Y.nslices > 1) slot_bytes += (size_t)Y.units * T * T * sizeof(float);
+        const size_t counters = (size_t)((X.sO == 0 ? 1 : nimg) * X.ntiles + (Y.sO == 0 ? 1 : nimg) * Y.ntiles);
+        if (slot_bytes <= ((size_t)256 << 20) && counters * sizeof(int) <= kCounterBytes) break;
+        len *= 2;
+    }
+    if (X.units + Y.units == 0) return LIBSHALOM_SUCCESS;
+    if ((long)X.units + Y.units > INT32_MAX) return LIBSHALOM_NOT_SUPPORTED;
+    if (X.nslices > 1 || Y.nslices > 1) {
+        int *counters;
+        float *slots;
+        st = fixup_space(dev, s, slot_bytes, &counters, &slots);
+        if (st) return st;
+        const long xt = X.nslices > 1 ? (long)(X.sO == 0 ? 1 : nimg) * X.ntiles : 0;
+        X.counters = counters;
+        Y.counters = counters + xt;
+        X.slots = slots;
+        Y.slots = slots + (X.nslices > 1 ? (size_t)X.units * T * T : 0);
+    }
+    BwdLaunch fn = bwd_table().fn[prec][ta][tb][big];
+    if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    return fn(X, Y, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
+}
 
 int sgemm_sync(int ta, int tb, float *C, float *A, float *B, long M, long N, long K, bool mp)
 {
@@ -588,32 +772,24 @@ int LibShalom_dgemm_batched(int transa, int transb, double *const *C, const doub
 int LibShalom_sgemm_backward(int transa, int transb, const float *dC, const float *A, const float *B, float *dA,
                              float *dB, long M, long N, long K, int precision, LibShalom_stream_t stream)
 {
+    return LibShalom_sgemm_backward_strided(transa, transb, dC, A, B, dA, dB, M, N, K, 1, 0, 0, 0, precision,
+                                            stream);
+}
+
+int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, const float *A, const float *B,
+                                     float *dA, float *dB, long M, long N, long K, long batch, long strideA,
+                                     long strideB, long strideC, int precision, LibShalom_stream_t stream)
+{
     int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
     if (!st) st = check_dims(transa, transb, M, N, K);
+    if (!st && (batch < 0 || batch > INT32_MAX || strideA < 0 || strideB < 0 || strideC < 0))
+        st = LIBSHALOM_INVALID_VALUE;
     if (st) return set_status(st);
-    if (M == 0 || N == 0 || K == 0) {
-        // dA / dB are M x K and K x N: zero-sized or all zeros.
-        if (K > 0 && M > 0 && dA && hipMemsetAsync(dA, 0, sizeof(float) * M * K, (hipStream_t)stream) != hipSuccess)
-            return set_status(LIBSHALOM_HIP_ERROR);
-        if (K > 0 && N > 0 && dB && hipMemsetAsync(dB, 0, sizeof(float) * K * N, (hipStream_t)stream) != hipSuccess)
-            return set_status(LIBSHALOM_HIP_ERROR);
-        return set_status(LIBSHALOM_SUCCESS);
-    }
-    if (!dC || !A || !B || !dA || !dB) return set_status(LIBSHALOM_INVALID_POINTER);
-    const int p = kprec(precision);
-    hipStream_t s = (hipStream_t)stream;
-    // op(A)' = dC op(B)^T  (M x K); stored transposed when transa.
-    if (!transa)
-        st = run_single(p, 0, 1 - transb, dA, dC, B, M, K, N, false, s);
-    else
-        st = run_single(p, transb, 1, dA, B, dC, K, M, N, false, s);
-    if (st) return set_status(st);
-    // op(B)' = op(A)^T dC  (K x N); stored transposed when transb.
-    if (!transb)
-        st = run_single(p, 1 - transa, 0, dB, A, dC, K, N, M, false, s);
-    else
-        st = run_single(p, 1, transa, dB, dC, A, N, K, M, false, s);
-    return set_status(st);
+    if (batch == 0) return set_status(LIBSHALOM_SUCCESS);
+    if ((M * N > 0 && !dC) || (M * K > 0 && (!A || !dA)) || (K * N > 0 && (!B || !dB)))
+        return set_status(LIBSHALOM_INVALID_POINTER);
+    return set_status(backward_run(kprec(precision), transa, transb, dC, A, B, dA, dB, M, N, K, (int)batch,
+                                   strideA, strideB, strideC, (hipStream_t)stream));
 }
 
 int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_cu, int *tile_m, int *tile_n,

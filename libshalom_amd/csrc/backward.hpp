@@ -1,0 +1,221 @@
+// backward.hpp -- the fused backward launch of C = op(A) op(B):
+//   op(A)' = dC op(B)^T   and   op(B)' = op(A)^T dC
+// for one GEMM or a strided batch of them, in ONE kernel launch.
+//
+// Each of the two gradients is a GEMM problem of its own ("X" = dA, "Y" =
+// dB): an output of R x Cc, a left operand R x L and a right operand L x Cc
+// (each stored plain or transposed), summed over a reduction of length L per
+// image -- and over the images too when the operand whose gradient it is is
+// shared by the batch (stride 0, e.g. the weights of a convolution).  Every
+// workgroup of the launch owns one unit: one output tile of X or of Y and one
+// slice of its reduction.  A slice is a run of segments (an image, or part of
+// an image's reduction) chained through the same accumulators.  When a tile's
+// reduction is cut into several slices, each slice writes its partial tile to
+// a slot, and the last slice to arrive (per-tile arrival counter) sums the
+// slots in slice order and writes the tile -- deterministic, inside the same
+// launch, and the counter is reset to zero for the next launch.
+//
+// The X and Y units that read the same dC block run in the same launch, so
+// dC is fetched from HBM about once and the second reader finds it in L2.
+#pragma once
+#include "kernels.hpp"
+
+namespace ls {
+
+struct BwdProblem {
+    const void *L, *Rt;  // left / right operand of image 0
+    void *O;             // output of image 0 (the only one when shared)
+    long long sL, sR, sO;  // per-image strides in elements; sO == 0: one output summed over images
+    int lda, ldb, ldo;
+    int R, Cc, red;      // output rows, columns; reduction length per image
+    int nimg;
+    int tiles_c, ntiles;  // tile grid of one output
+    int nslices;          // reduction slices per output tile
+    int spi;              // > 0: slices per image (a slice lies inside one image)
+    int gimg;             // spi == 0: whole images per slice
+    int slen;             // spi > 0: reduction length of a slice (multiple of the K chunk)
+    int units;            // workgroups of this problem
+    float *slots;         // nslices > 1: partial tiles, [tile][slice][BM * BN]
+    int *counters;        // nslices > 1: arrivals per tile (zero between launches)
+};
+
+// The segments of one unit's slice as a job stream (see Tile::stream).  The
+// problem's fields are copied in (scalar registers): a pointer to the kernel
+// argument would send it through private memory.
+template <class Cfg>
+struct SliceSrc {
+    const void *L, *Rt;
+    long long sL, sR;
+    int lda, ldb;
+    int R, Cc;
+    int img0, nseg;   // first image and number of segments
+    int rho0, rho1;   // reduction range of every segment
+    int r0, c0;       // output tile origin
+    void *out;        // where the tile goes: the output, or the unit's slot
+    int ldo;
+
+    __device__ __forceinline__ int count() const { return nseg; }
+    __device__ __forceinline__ GemmDesc get(int j) const
+    {
+        typedef typename Cfg::T T;
+        const int img = img0 + j;
+        const T *Li = (const T *)L + (long long)img * sL;
+        const T *Ri = (const T *)Rt + (long long)img * sR;
+        GemmDesc d;
+        d.A = Li + (Cfg::TA ? (long long)rho0 * lda + r0 : (long long)r0 * lda + rho0);
+        d.B = Ri + (Cfg::TB ? (long long)c0 * ldb + rho0 : (long long)rho0 * ldb + c0);
+        d.C = out;
+        d.M = min(Cfg::BM, R - r0);
+        d.N = min(Cfg::BN, Cc - c0);
+        d.K = rho1 - rho0;
+        d.lda = lda;
+        d.ldb = ldb;
+        d.ldc = ldo;
+        d.flags = j + 1 < nseg ? kChain : 0;
+        return d;
+    }
+};
+
+// Sum the slots of tile tg in slice order into the output (last arrival).
+template <int BM, int BN>
+__device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_out, int r0, int c0, int *flag)
+{
+    __threadfence();  // this workgroup's partial tile is visible device-wide
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int old = atomicAdd(&P.counters[tg], 1);
+        *flag = old == P.nslices - 1;
+    }
+    __syncthreads();
+    if (!*flag) return;
+    __threadfence();  // and the other slices' tiles are visible here
+    const int tr = min(BM, P.R - r0), tcn = min(BN, P.Cc - c0);
+    const float *base = P.slots + (size_t)tg * P.nslices * (BM * BN);
+    float *O = (float *)P.O + (long long)img_out * P.sO;
+    for (int e = threadIdx.x * 4; e < tr * BN; e += kThreads * 4) {
+        const int r = e / BN, c = e % BN;
+        if (c >= tcn) continue;
+        f32x4 acc = *(const f32x4 *)(base + e);
+        for (int s = 1; s < P.nslices; ++s) acc += *(const f32x4 *)(base + (size_t)s * (BM * BN) + e);
+        float *o = O + (long long)(r0 + r) * P.ldo + c0 + c;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (c + q < tcn) o[q] = acc[q];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) P.counters[tg] = 0;  // ready for the next launch
+}
+
+template <class Cfg>
+__device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, int *flag)
+{
+    const int tg = u / P.nslices, s = u - tg * P.nslices;
+    const bool shared = P.sO == 0;
+    const int img_out = shared ? 0 : tg / P.ntiles;
+    const int t = shared ? tg : tg - img_out * P.ntiles;
+    const int tr = t / P.tiles_c, tc = t - tr * P.tiles_c;
+    SliceSrc<Cfg> src;
+    src.L = P.L;
+    src.Rt = P.Rt;
+    src.sL = P.sL;
+    src.sR = P.sR;
+    src.lda = P.lda;
+    src.ldb = P.ldb;
+    src.R = P.R;
+    src.Cc = P.Cc;
+    src.r0 = tr * Cfg::BM;
+    src.c0 = tc * Cfg::BN;
+    if (!shared) {
+        src.img0 = img_out;
+        src.nseg = 1;
+        src.rho0 = s * P.slen;
+        src.rho1 = min(P.red, src.rho0 + P.slen);
+    } else if (P.spi > 0) {
+        const int img = s / P.spi, sub = s - img * P.spi;
+        src.img0 = img;
+        src.nseg = 1;
+        src.rho0 = sub * P.slen;
+        src.rho1 = min(P.red, src.rho0 + P.slen);
+    } else {
+        src.img0 = s * P.gimg;
+        src.nseg = min(P.gimg, P.nimg - src.img0);
+        src.rho0 = 0;
+        src.rho1 = P.red;
+    }
+    if (P.nslices == 1) {
+        src.out = (float *)P.O + (long long)img_out * P.sO + (long long)src.r0 * P.ldo + src.c0;
+        src.ldo = P.ldo;
+    } else {
+        src.out = P.slots + ((size_t)tg * P.nslices + s) * (Cfg::BM * Cfg::BN);
+        src.ldo = Cfg::BN;
+    }
+    Tile<Cfg> tile;
+    tile.stream(src, 0, 1, lds);
+    if (P.nslices > 1) bwd_fixup<Cfg::BM, Cfg::BN>(P, tg, img_out, src.r0, src.c0, flag);
+}
+
+template <class CX, class CY>
+struct BwdLds {
+    static constexpr int BYTES = CX::LDS_BYTES > CY::LDS_BYTES ? CX::LDS_BYTES : CY::LDS_BYTES;
+};
+
+// Units [0, X.units) are dA units, the rest dB units.
+template <class CX, class CY>
+__global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads))) void
+k_gemm_backward(BwdProblem X, BwdProblem Y)
+{
+    __shared__ __attribute__((aligned(16))) char lds[BwdLds<CX, CY>::BYTES];
+    __shared__ int flag;
+    const int u = blockIdx.x;
+    if (u < X.units)
+        bwd_unit<CX>(X, u, lds, &flag);
+    else
+        bwd_unit<CY>(Y, u - X.units, lds, &flag);
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------
+typedef hipError_t (*BwdLaunch)(const BwdProblem &X, const BwdProblem &Y, hipStream_t s);
+
+// (transa, transb) of the forward product -> (TA, TB) of the dA and dB
+// problems (left operand transposed / right operand transposed).
+constexpr int kBwdTA[2][2][2] = {{{0, 1}, {0, 1}}, {{0, 0}, {1, 1}}};  // [ta][tb][X/Y]: left transposed
+constexpr int kBwdTB[2][2][2] = {{{1, 0}, {0, 0}}, {{1, 0}, {1, 1}}};  // [ta][tb][X/Y]: right transposed
+
+template <int PREC, int TA, int TB, int BT>
+hipError_t launch_backward(const BwdProblem &X, const BwdProblem &Y, hipStream_t s)
+{
+    typedef TileCfg<PREC, kBwdTA[TA][TB][0], kBwdTB[TA][TB][0], BT, BT> CX;
+    typedef TileCfg<PREC, kBwdTA[TA][TB][1], kBwdTB[TA][TB][1], BT, BT> CY;
+    const int grid = X.units + Y.units;
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_gemm_backward<CX, CY>), dim3(grid), dim3(kThreads), 0, s, X, Y);
+    return hipGetLastError();
+}
+
+constexpr int NBWD_TILE = 2;  // 0: 64 x 64, 1: 128 x 128
+struct BwdTable {
+    BwdLaunch fn[NPREC][2][2][NBWD_TILE];
+};
+BwdTable &bwd_table();
+
+template <int PREC>
+void register_backward(BwdTable &t)
+{
+    t.fn[PREC][0][0][0] = &launch_backward<PREC, 0, 0, 64>;
+    t.fn[PREC][0][1][0] = &launch_backward<PREC, 0, 1, 64>;
+    t.fn[PREC][1][0][0] = &launch_backward<PREC, 1, 0, 64>;
+    t.fn[PREC][1][1][0] = &launch_backward<PREC, 1, 1, 64>;
+    t.fn[PREC][0][0][1] = &launch_backward<PREC, 0, 0, 128>;
+    t.fn[PREC][0][1][1] = &launch_backward<PREC, 0, 1, 128>;
+    t.fn[PREC][1][0][1] = &launch_backward<PREC, 1, 0, 128>;
+    t.fn[PREC][1][1][1] = &launch_backward<PREC, 1, 1, 128>;
+}
+
+void register_backward_fp32(BwdTable &t);
+void register_backward_bf16x3(BwdTable &t);
+void register_backward_bf16(BwdTable &t);
+void register_backward_bf16x6(BwdTable &t);
+
+}  // namespace ls

@@ -51,8 +51,13 @@ struct GemmDesc {
     void *C;
     int M, N, K;
     int lda, ldb, ldc;
-    int flags;  // reserved (0)
+    int flags;  // kChain: the next job of the workgroup's stream continues this accumulator
 };
+
+// GemmDesc::flags bit: the job is one segment of a longer reduction.  Its C
+// tile is not stored and the next job of the same stream (which must have the
+// same tile) adds to the same accumulators.
+constexpr int kChain = 1;
 
 template <int PREC> struct PrecInfo;
 template <> struct PrecInfo<P_FP32> { typedef float T; static constexpr int planes = 0; };
@@ -549,9 +554,11 @@ struct Tile {
     {
         load_chunk(a0, b0, n2.d, n2.c * BK, n2.job >= 0, tid);
         compute(lds_buf(lds, bi), lane, ksteps(cur.d.K, cur.c * BK));
-        if (cur.c + 1 == cur.nch) store((T *)cur.d.C, cur.d.ldc, cur.d.M, cur.d.N);
+        const bool chained = (cur.d.flags & kChain) != 0;
+        if (cur.c + 1 == cur.nch && !chained) store((T *)cur.d.C, cur.d.ldc, cur.d.M, cur.d.N);
         if (n1.job < 0) return false;
-        if (n1.c == 0) begin(n1.d.M, n1.d.N, 0, 0, wave);
+        // n1 starting a job means cur ends one: a chained job keeps its sums
+        if (n1.c == 0 && !chained) begin(n1.d.M, n1.d.N, 0, 0, wave);
         if constexpr (Cfg::NBUF == 1) __syncthreads();
         chunk_to_lds(a1, b1, lds_buf(lds, bi + 1), tid);
         __syncthreads();

@@ -58,6 +58,74 @@ def test_argument_errors_before_device(args, code):
     assert e.value.status == code
 
 
+@pytest.mark.parametrize("kw,code", [
+    (dict(batch=-1), 1),
+    (dict(strideA=-1), 1),
+    (dict(strideC=-5), 1),
+    (dict(M=-1), 1),
+    (dict(transa=3), 1),
+    (dict(precision=ls.FP64), 1),
+    (dict(dC=None), 2),
+    (dict(dA=None), 2),
+    (dict(B=None), 2),
+])
+def test_backward_argument_errors_before_device(kw, code):
+    args = dict(transa=0, transb=0, dC=0x1000, A=0x1000, B=0x1000, dA=0x1000, dB=0x1000, M=4, N=4, K=4,
+                batch=2, strideA=0, strideB=16, strideC=16, precision=ls.FP32)
+    args.update(kw)
+    L = ls.lib()
+    st = L.LibShalom_sgemm_backward_strided(args["transa"], args["transb"], args["dC"], args["A"], args["B"],
+                                            args["dA"], args["dB"], args["M"], args["N"], args["K"],
+                                            args["batch"], args["strideA"], args["strideB"], args["strideC"],
+                                            args["precision"], None)
+    assert st == code and L.LibShalom_get_last_error() == code
+    if args["batch"] == 2 and args["strideA"] == 0 and args["strideC"] == 16:
+        st = L.LibShalom_sgemm_backward(args["transa"], args["transb"], args["dC"], args["A"], args["B"],
+                                        args["dA"], args["dB"], args["M"], args["N"], args["K"],
+                                        args["precision"], None)
+        assert st == code
+
+
+def test_backward_empty_batch_is_a_no_op():
+    L = ls.lib()
+    assert L.LibShalom_sgemm_backward_strided(0, 0, None, None, None, None, None, 4, 4, 4, 0, 0, 0, 0,
+                                              ls.FP32, None) == 0
+
+
+def test_backward_transpose_tables_match_derivation():
+    """The (TA, TB) that the dA and dB problems use, restated from the chain
+    rule: dA = dC op(B)^T (stored transposed when transa), dB = op(A)^T dC
+    (stored transposed when transb); checked with explicit matrices."""
+    import numpy as np
+    rng = np.random.default_rng(0)
+    M, N, K = 3, 4, 5
+    for ta in (0, 1):
+        for tb in (0, 1):
+            A = rng.standard_normal((K, M) if ta else (M, K))
+            B = rng.standard_normal((N, K) if tb else (K, N))
+            dC = rng.standard_normal((M, N))
+            opA = A.T if ta else A
+            opB = B.T if tb else B
+            # finite-difference-free check of the closed forms on a bilinear loss
+            dA = dC @ opB.T
+            dB = opA.T @ dC
+            dA = dA.T if ta else dA
+            dB = dB.T if tb else dB
+            eps = 1e-6
+            E = np.zeros_like(A)
+            E[0, 1] = 1.0
+            Ap = A + eps * E
+            opAp = Ap.T if ta else Ap
+            fd = (np.sum(dC * (opAp @ opB)) - np.sum(dC * (opA @ opB))) / eps
+            assert abs(fd - dA[0, 1]) < 1e-5
+            E = np.zeros_like(B)
+            E[1, 0] = 1.0
+            Bp = B + eps * E
+            opBp = Bp.T if tb else Bp
+            fd = (np.sum(dC * (opA @ opBp)) - np.sum(dC * (opA @ opB))) / eps
+            assert abs(fd - dB[1, 0]) < 1e-5
+
+
 def test_bad_precision_rejected():
     with pytest.raises(ls.LibShalomError):
         ls.set_precision(9)
