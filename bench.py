@@ -10,6 +10,11 @@ way and reported beside it, each with its max / mean componentwise relative
 error |C - C64| / (|A||B|) against an FP64 product of a sample of the GEMMs.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --workload {conv,mp,sweep,hello,dgemm_batched}
+The other workloads print one JSON line each of their own (the batched SGEMM
+line above is the headline): the im2col convolution GEMMs forward + fused
+backward (batch 32), tall-skinny / short-wide sgemm_mp, the cubic 8..128
+sweep, the 80^3 NT call, and 100k batched CP2K-sized DGEMMs.
 """
 import argparse
 import json
@@ -38,6 +43,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--modes", default="bf16x3,fp32,bf16x6,bf16",
                     help="precision modes to time besides the headline bf16x3")
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "conv", "mp", "sweep", "hello", "dgemm_batched"])
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="issue one device-to-device copy of this many MiB first (untimed), a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -54,6 +61,9 @@ def main():
     from libshalom_amd.workloads import uniform_shapes
 
     dev = torch.device("cuda:0")
+    if args.workload != "batched":
+        OTHER[args.workload](args, dev)
+        return
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib * (1 << 20) // 4
         src = torch.ones(n, device=dev)
@@ -252,6 +262,289 @@ def cpu_baseline(shapes, seconds):
     return {"value": round(gflops, 2), "unit": "GFLOP/s", "cores": oracle.cpu_threads(), "kind": "port",
             "sample": f"{n} GEMMs of the batch ({tot / 1e9:.2f} GFLOP) x {reps} reps, FP32 OpenMP triple loop "
                       f"(i-k-j), gcc -O3 -fopenmp -march=x86-64-v3"}
+
+
+# ---------------------------------------------------------------------------
+# Other workloads (one JSON line each).
+# ---------------------------------------------------------------------------
+def time_ms(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        fn()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def roof_tflops(mode, flops, nbytes):
+    """min(MFMA peak of the mode, arithmetic intensity x HBM peak), TFLOP/s."""
+    return min(PEAK_TFLOPS[mode], flops / nbytes * PEAK_HBM_GBS / 1e3)
+
+
+def rel_err(got, ref, den):
+    e = (got.double() - ref).abs()
+    r = e / den.clamp_min(1e-300)
+    return float(r.max()), float(r.mean()), e, den
+
+
+def emit(obj):
+    print(json.dumps(obj), flush=True)
+
+
+def bench_conv(args, dev):
+    """im2col convolution layers, batch 32: forward C_i = W X_i (one batched
+    plan, W shared) and the fused backward (dW summed over the batch, dX per
+    image) in one launch.  W ~ N(0, 2/K), X and dC ~ N(0, 1)."""
+    import torch
+
+    import libshalom_amd as ls
+    from libshalom_amd.workloads import CONV_SHAPES
+    stream = torch.cuda.current_stream()
+    batch = 32
+    layers, tot_f, tot_ms = [], {}, {}
+    for (Cout, HW, Kc) in CONV_SHAPES:
+        g = torch.Generator(device=dev).manual_seed(Cout)
+        W = torch.randn(Cout, Kc, generator=g, device=dev) * (2.0 / Kc) ** 0.5
+        X = torch.randn(batch, Kc, HW, generator=g, device=dev)
+        dY = torch.randn(batch, Cout, HW, generator=g, device=dev)
+        Y = torch.empty(batch, Cout, HW, device=dev)
+        dW = torch.empty_like(W)
+        dX = torch.empty_like(X)
+        ff = 2.0 * Cout * HW * Kc * batch
+        fb = 2 * ff
+        bf = 4.0 * (Cout * Kc + batch * Kc * HW + batch * Cout * HW)
+        bb = 4.0 * (batch * Cout * HW + 2 * Cout * Kc + 2 * batch * Kc * HW)
+        row = {"M": Cout, "N": HW, "K": Kc, "batch": batch}
+        for name, prec in (("fp32", ls.FP32), ("bf16x3", ls.BF16X3)):
+            plan = ls.BatchedPlan(0, 0, [Y[i] for i in range(batch)], [W] * batch, [X[i] for i in range(batch)],
+                                  [Cout] * batch, [HW] * batch, [Kc] * batch, precision=prec)
+            fms = time_ms(lambda: plan.execute(stream), args.steps, args.warmup, stream)
+            bms = time_ms(lambda: ls.sgemm_backward_strided(0, 0, dY, W, X, dW, dX, Cout, HW, Kc, batch, 0,
+                                                            Kc * HW, Cout * HW, prec, stream),
+                          args.steps, args.warmup, stream)
+            plan.close()
+            # errors: forward and dX on image 0, dW in full
+            W64, X0, dY64 = W.double(), X[0].double(), dY.double()
+            fe = rel_err(Y[0], W64 @ X0, W64.abs() @ X0.abs())
+            gW = sum(dY64[i] @ X[i].double().T for i in range(batch))
+            bW = sum(dY64[i].abs() @ X[i].double().abs().T for i in range(batch))
+            we = rel_err(dW, gW, bW)
+            xe = rel_err(dX[0], W64.T @ dY64[0], W64.abs().T @ dY64[0].abs())
+            row[name] = {
+                "fwd_ms": round(fms, 4), "bwd_ms": round(bms, 4),
+                "fwd_tflops": round(ff / fms / 1e9, 2), "bwd_tflops": round(fb / bms / 1e9, 2),
+                "fwd_roof_frac": round(ff / fms / 1e9 / roof_tflops(name, ff, bf), 3),
+                "bwd_roof_frac": round(fb / bms / 1e9 / roof_tflops(name, fb, bb), 3),
+                "fwd_max_rel": float(f"{fe[0]:.3g}"), "dW_max_rel": float(f"{we[0]:.3g}"),
+                "dX_max_rel": float(f"{xe[0]:.3g}"),
+            }
+            tot_f[name] = tot_f.get(name, 0.0) + ff + fb
+            tot_ms[name] = tot_ms.get(name, 0.0) + fms + bms
+            log(f"conv {Cout}x{HW}x{Kc} {name}: fwd {fms:.3f} ms ({ff / fms / 1e9:.1f} TF/s), "
+                f"bwd {bms:.3f} ms ({fb / bms / 1e9:.1f} TF/s), max rel fwd {fe[0]:.2g} dW {we[0]:.2g} "
+                f"dX {xe[0]:.2g}")
+        layers.append(row)
+        del W, X, dY, Y, dW, dX
+        torch.cuda.empty_cache()
+    emit({"metric": "im2col conv GEMM forward + fused backward GFLOP/s, batch 32 (four layers)",
+          "value": round(tot_f["bf16x3"] / tot_ms["bf16x3"] / 1e6, 1), "unit": "GFLOP/s",
+          "fp32_gflops": round(tot_f["fp32"] / tot_ms["fp32"] / 1e6, 1), "n_gpus": 1,
+          "dtype": "bf16x3 (value) and fp32", "data": "synthetic: W~N(0,2/K), X,dC~N(0,1)",
+          "config": {"workload": "conv layers (M=Cout,N=HW,K=Cin*9) x batch 32"}, "layers": layers})
+
+
+def bench_mp(args, dev):
+    """LibShalom_sgemm_mp shapes: tall-skinny M x N x K and short-wide
+    transposes, NN and NT, FP32 vs BF16X3, per-shape GFLOP/s, fraction of
+    min(MFMA peak, AI x HBM peak), and max / mean relative error on 64
+    sampled rows against FP64."""
+    import torch
+
+    import libshalom_amd as ls
+    from libshalom_amd.workloads import TALL_SKINNY
+    stream = torch.cuda.current_stream()
+    g = torch.Generator(device=dev).manual_seed(0)
+    big = torch.rand(65536 * 1024 + 64, generator=g, device=dev).mul_(2).sub_(1)
+    bigB = torch.rand(65536 * 1024 + 64, generator=g, device=dev).mul_(2).sub_(1)
+    Cbuf = torch.empty(65536 * 1024, device=dev)
+    shapes = [(M, N, K, "tall") for (M, N, K) in TALL_SKINNY]
+    shapes += [(N, M, K, "wide") for (M, N, K) in TALL_SKINNY]  # short-wide: M small, N large
+    rows = []
+    rng = torch.Generator().manual_seed(1)
+    for (M, N, K, kind) in shapes:
+        A = big[:M * K].view(M, K)
+        for tb in (0, 1):
+            B = bigB[:K * N].view(N, K) if tb else bigB[:K * N].view(K, N)
+            C = Cbuf[:M * N].view(M, N)
+            opB = B.double().T if tb else B.double()
+            sel = torch.randint(0, M, (min(64, M),), generator=rng).to(dev)
+            row = {"M": M, "N": N, "K": K, "tb": tb, "kind": kind}
+            flops = 2.0 * M * N * K
+            nbytes = 4.0 * (M * K + K * N + M * N)
+            for name, prec in (("fp32", ls.FP32), ("bf16x3", ls.BF16X3)):
+                ms = time_ms(lambda: ls.sgemm_mp_ex(0, tb, C, A, B, M, N, K, prec, stream), args.steps,
+                             args.warmup, stream)
+                a = A[sel].double()
+                e = rel_err(C[sel], a @ opB, a.abs() @ opB.abs())
+                tf = flops / ms / 1e9
+                row[name] = {"ms": round(ms, 5), "tflops": round(tf, 2),
+                             "roof_frac": round(tf / roof_tflops(name, flops, nbytes), 3),
+                             "max_rel": float(f"{e[0]:.3g}"), "mean_rel": float(f"{e[1]:.3g}")}
+            rows.append(row)
+            log(f"mp {kind} {M}x{N}x{K} tb={tb}: fp32 {row['fp32']['tflops']} TF/s "
+                f"({row['fp32']['roof_frac']}), bf16x3 {row['bf16x3']['tflops']} TF/s "
+                f"({row['bf16x3']['roof_frac']}), rel err fp32 {row['fp32']['max_rel']} "
+                f"bf16x3 {row['bf16x3']['max_rel']}")
+    tf = sum(2.0 * r["M"] * r["N"] * r["K"] for r in rows)
+    t32 = sum(r["fp32"]["ms"] for r in rows)
+    t3 = sum(r["bf16x3"]["ms"] for r in rows)
+    emit({"metric": "sgemm_mp tall-skinny / short-wide GFLOP/s (all shapes, NN+NT)",
+          "value": round(tf / t3 / 1e6, 1), "unit": "GFLOP/s", "fp32_gflops": round(tf / t32 / 1e6, 1),
+          "n_gpus": 1, "dtype": "bf16x3 (value) and fp32", "data": "synthetic: U[-1,1)",
+          "config": {"workload": "M in {4096,16384,65536} x N in {16,32,64} x K in {64,256,1024} and "
+                                 "the transposed short-wide shapes"}, "shapes": rows})
+
+
+def bench_sweep(args, dev):
+    """Cubic n = 8..128 step 8, the four NoTrans/Trans combinations, FP32:
+    one LibShalom_sgemm_ex call per size, 100 timed calls after 5 warm-ups."""
+    import torch
+
+    import libshalom_amd as ls
+    from libshalom_amd.workloads import SMALL_SWEEP
+    stream = torch.cuda.current_stream()
+    rows = []
+    g = torch.Generator(device=dev).manual_seed(0)
+    for (n, _, _) in SMALL_SWEEP:
+        A = torch.rand(n, n, generator=g, device=dev).mul_(2).sub_(1)
+        B = torch.rand(n, n, generator=g, device=dev).mul_(2).sub_(1)
+        C = torch.empty(n, n, device=dev)
+        row = {"n": n}
+        for ta in (0, 1):
+            for tb in (0, 1):
+                ms = time_ms(lambda: ls.sgemm_ex(ta, tb, C, A, B, n, n, n, ls.FP32, stream), 100, 5, stream)
+                row[f"{'NT'[ta]}{'NT'[tb]}"] = {"us": round(ms * 1e3, 2),
+                                                "gflops": round(2.0 * n ** 3 / ms / 1e6, 2)}
+        rows.append(row)
+        log(f"sweep n={n}: " + " ".join(f"{k} {v['us']}us" for k, v in row.items() if k != "n"))
+    cpu = None
+    if not args.no_cpu:
+        import numpy as np
+
+        import oracle
+        cpu = []
+        for (n, _, _) in SMALL_SWEEP:
+            A = np.random.default_rng(n).random((n, n), dtype=np.float32)
+            t0 = time.perf_counter()
+            for _ in range(100):
+                oracle.cpu_sgemm(0, 0, A, A, n, n, n)
+            el = (time.perf_counter() - t0) / 100
+            cpu.append({"n": n, "us": round(el * 1e6, 2), "gflops": round(2.0 * n ** 3 / el / 1e9, 2)})
+        cpu = {"kind": "port", "cores": oracle.cpu_threads(), "sizes": cpu}
+    emit({"metric": "small SGEMM sweep: per-call time, n=8..128 step 8, FP32", "value": rows[-1]["NN"]["gflops"],
+          "unit": "GFLOP/s at n=128 NN", "n_gpus": 1, "dtype": "fp32", "data": "synthetic: U[-1,1)",
+          "config": {"workload": "cubic n=8..128 step 8 x {NN,NT,TN,TT}, 5 warm-ups + 100 calls"},
+          "sizes": rows, "cpu_baseline": cpu})
+
+
+def bench_hello(args, dev):
+    """The 80^3 NoTrans/Trans call with the reference harness's inputs."""
+    import torch
+
+    import libshalom_amd as ls
+    from libshalom_amd.workloads import hello_inputs
+    A, B = hello_inputs(80, 80, 80)
+    At = torch.from_numpy(A).to(dev)
+    Bt = torch.from_numpy(B).to(dev)
+    C = torch.full((80, 80), float("nan"), device=dev)
+    stream = torch.cuda.current_stream()
+    ms = time_ms(lambda: ls.sgemm_ex(0, 1, C, At, Bt, 80, 80, 80, ls.FP32, stream), 100, 5, stream)
+    t0 = time.perf_counter()
+    for _ in range(100):
+        ls.sgemm(0, 1, C, At, Bt, 80, 80, 80)
+    sync_us = (time.perf_counter() - t0) / 100 * 1e6
+    a = At.double().view(80, 80)
+    b = Bt.double().view(80, 80).T
+    mx, mean, e, den = rel_err(C, a @ b, a.abs() @ b.abs())
+    viol = int((e > 2 * 80 * 2.0 ** -24 * den).sum())
+    emit({"metric": "Hello 80x80x80 NT sgemm", "value": round(ms * 1e3, 2), "unit": "us per call (stream)",
+          "higher_is_better": False, "sync_api_us": round(sync_us, 2), "gflops": round(1.024e6 / ms / 1e6, 2),
+          "violations": viol, "max_rel": float(f"{mx:.3g}"), "n_gpus": 1, "dtype": "fp32",
+          "data": "drand48 default-state inputs (reference harness recipe)",
+          "config": {"workload": "LibShalom_sgemm(NoTrans, Trans) M=N=K=80"}})
+
+
+def bench_dgemm_batched(args, dev):
+    """100,000 DGEMMs with M, N, K drawn from the small-block dims
+    {4,5,9,13,16,22,23,26,32}, FP64 on FP64 MFMA, one batched plan."""
+    import numpy as np
+    import torch
+
+    import libshalom_amd as ls
+    from libshalom_amd.workloads import cp2k_shapes
+    count = 100000
+    shapes = cp2k_shapes(count, seed=0)
+    Ms = np.array([s[0] for s in shapes], dtype=np.int64)
+    Ns = np.array([s[1] for s in shapes], dtype=np.int64)
+    Ks = np.array([s[2] for s in shapes], dtype=np.int64)
+    offA = np.concatenate([[0], np.cumsum(Ms * Ks)])
+    offB = np.concatenate([[0], np.cumsum(Ks * Ns)])
+    offC = np.concatenate([[0], np.cumsum(Ms * Ns)])
+    g = torch.Generator(device=dev).manual_seed(0)
+    dA = torch.rand(int(offA[-1]) + 8, generator=g, device=dev, dtype=torch.float64).mul_(2).sub_(1)
+    dB = torch.rand(int(offB[-1]) + 8, generator=g, device=dev, dtype=torch.float64).mul_(2).sub_(1)
+    dC = torch.zeros(int(offC[-1]) + 8, device=dev, dtype=torch.float64)
+    As = [dA.data_ptr() + 8 * int(o) for o in offA[:-1]]
+    Bs = [dB.data_ptr() + 8 * int(o) for o in offB[:-1]]
+    Cs = [dC.data_ptr() + 8 * int(o) for o in offC[:-1]]
+    flops = float(np.sum(2.0 * Ms * Ns * Ks))
+    nbytes = float(np.sum(8.0 * (Ms * Ks + Ks * Ns + Ms * Ns)))
+    stream = torch.cuda.current_stream()
+    plan = ls.BatchedPlan(0, 0, Cs, As, Bs, Ms, Ns, Ks, fp64=True)
+    ms = time_ms(lambda: plan.execute(stream), args.steps, args.warmup, stream)
+    plan.close()
+    mx = 0.0
+    for i in np.random.default_rng(1).choice(count, 300, replace=False):
+        m, n, k = int(Ms[i]), int(Ns[i]), int(Ks[i])
+        a = dA[offA[i]:offA[i] + m * k].view(m, k)
+        b = dB[offB[i]:offB[i] + k * n].view(k, n)
+        c = dC[offC[i]:offC[i] + m * n].view(m, n)
+        mx = max(mx, rel_err(c, a @ b, a.abs() @ b.abs())[0])
+    gbs = nbytes / ms / 1e6
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        sel = list(range(2000))
+        Ah = [np.random.default_rng(i).random((int(Ms[i]), int(Ks[i]))) for i in sel]
+        Bh = [np.random.default_rng(i + 1).random((int(Ks[i]), int(Ns[i]))) for i in sel]
+        Ch = [np.zeros((int(Ms[i]), int(Ns[i]))) for i in sel]
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 3.0:
+            oracle.cpu_gemm_batched(0, 0, Ch, Ah, Bh, Ms[sel], Ns[sel], Ks[sel])
+            reps += 1
+        el = time.perf_counter() - t0
+        sf = float(np.sum(2.0 * Ms[sel] * Ns[sel] * Ks[sel]))
+        cpu = {"value": round(sf * reps / el / 1e9, 2), "unit": "GFLOP/s", "cores": oracle.cpu_threads(),
+               "kind": "port", "sample": f"first 2000 GEMMs x {reps} reps, FP64 OpenMP triple loop"}
+    emit({"metric": "batched DGEMM GFLOP/s, 100k GEMMs with CP2K block dims", "value": round(flops / ms / 1e6, 1),
+          "unit": "GFLOP/s", "ms_per_step": round(ms, 4), "hbm_gbs": round(gbs, 1),
+          "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": round(gbs / PEAK_HBM_GBS, 4)},
+          "max_rel_err": float(f"{mx:.3g}"), "n_gpus": 1, "dtype": "f64", "data": "synthetic: U[-1,1)",
+          "config": {"workload": "100k DGEMMs, M,N,K from {4,5,9,13,16,22,23,26,32}, NN",
+                     "gflop_per_step": round(flops / 1e9, 4), "gb_per_step": round(nbytes / 1e9, 4)},
+          "cpu_baseline": cpu})
+
+
+OTHER = {"conv": bench_conv, "mp": bench_mp, "sweep": bench_sweep, "hello": bench_hello,
+         "dgemm_batched": bench_dgemm_batched}
 
 
 if __name__ == "__main__":

@@ -174,3 +174,12 @@ def test_cpu_baseline_batched_matches_single():
                 assert np.array_equal(Cs[g], single)
                 cref, s = oracle.sgemm_ref(ta, tb, As[g], Bs[g], m, n, k)
                 assert oracle.check(single, cref, s, k)[0] == 0
+
+
+def test_workload_hello_inputs_match_oracle():
+    """The package's drand48 input generator (used by bench.py) reproduces the
+    oracle's Hello.c inputs bit for bit."""
+    from libshalom_amd.workloads import hello_inputs
+    A, B = hello_inputs(80, 80, 80)
+    A0, B0 = oracle.hello_inputs(80, 80, 80)
+    assert np.array_equal(A, A0) and np.array_equal(B, B0)
