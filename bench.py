@@ -298,9 +298,9 @@ def emit(obj):
 
 
 def bench_conv(args, dev):
-    """im2col convolution layers, batch 32: forward C_i = W X_i (one batched
-    plan, W shared) and the fused backward (dW summed over the batch, dX per
-    image) in one launch.  W ~ N(0, 2/K), X and dC ~ N(0, 1)."""
+    """im2col convolution layers, batch 32: forward C_i = W X_i (one strided
+    batched launch, W shared) and the fused backward (dW summed over the
+    batch, dX per image) in one launch.  W ~ N(0, 2/K), X and dC ~ N(0, 1)."""
     import torch
 
     import libshalom_amd as ls
@@ -322,13 +322,12 @@ def bench_conv(args, dev):
         bb = 4.0 * (batch * Cout * HW + 2 * Cout * Kc + 2 * batch * Kc * HW)
         row = {"M": Cout, "N": HW, "K": Kc, "batch": batch}
         for name, prec in (("fp32", ls.FP32), ("bf16x3", ls.BF16X3)):
-            plan = ls.BatchedPlan(0, 0, [Y[i] for i in range(batch)], [W] * batch, [X[i] for i in range(batch)],
-                                  [Cout] * batch, [HW] * batch, [Kc] * batch, precision=prec)
-            fms = time_ms(lambda: plan.execute(stream), args.steps, args.warmup, stream)
+            fms = time_ms(lambda: ls.sgemm_strided_batched(0, 0, Y, W, X, Cout, HW, Kc, batch, 0, Kc * HW,
+                                                           Cout * HW, prec, stream),
+                          args.steps, args.warmup, stream)
             bms = time_ms(lambda: ls.sgemm_backward_strided(0, 0, dY, W, X, dW, dX, Cout, HW, Kc, batch, 0,
                                                             Kc * HW, Cout * HW, prec, stream),
                           args.steps, args.warmup, stream)
-            plan.close()
             # errors: forward and dX on image 0, dW in full
             W64, X0, dY64 = W.double(), X[0].double(), dY.double()
             fe = rel_err(Y[0], W64 @ X0, W64.abs() @ X0.abs())

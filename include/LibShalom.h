@@ -125,6 +125,17 @@ int LibShalom_dgemm_batched(int transa, int transb, double *const *C, const doub
                             const double *const *B, const long *M, const long *N, const long *K,
                             long count, LibShalom_stream_t stream);
 
+/* Strided batch of GEMMs in one launch: C_i = op(A_i) op(B_i) with
+ * X_i = X + i*strideX.  strideA or strideB 0: the operand is shared by the
+ * batch (e.g. the weights of an im2col convolution); strideC 0: C is the sum
+ * over the batch.  Long reductions are cut into slices whose partial tiles
+ * are summed inside the launch in a fixed order.  FP32-input modes only.  No
+ * reference counterpart (the reference has no batched entry point). */
+int LibShalom_sgemm_strided_batched(int transa, int transb, float *C, const float *A,
+                                    const float *B, long M, long N, long K, long batch,
+                                    long strideA, long strideB, long strideC, int precision,
+                                    LibShalom_stream_t stream);
+
 /* Backward of C = op(A) op(B) given dC (M x N): writes dA with the storage
  * shape of A and dB with the storage shape of B,
  *   op(A)' = dC op(B)^T,   op(B)' = op(A)^T dC,

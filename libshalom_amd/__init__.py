@@ -21,7 +21,7 @@ __all__ = [
     "LibShalomError", "lib", "sgemm", "sgemm_mp", "dgemm", "sgemm_ex", "sgemm_mp_ex",
     "dgemm_ex", "set_thread_nums", "set_thread_ways", "get_thread_config",
     "set_precision", "get_precision", "BatchedPlan", "sgemm_batched", "dgemm_batched",
-    "sgemm_backward", "sgemm_backward_strided", "query_plan", "query_batched_tile", "LIB_PATH",
+    "sgemm_backward", "sgemm_backward_strided", "sgemm_strided_batched", "query_plan", "query_batched_tile", "LIB_PATH",
 ]
 
 NoTrans = 0
@@ -80,6 +80,7 @@ SIGNATURES = {
     "LibShalom_sgemm_batched": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _L, _I, _P]),
     "LibShalom_dgemm_batched": (_I, [_I, _I, _P, _P, _P, _P, _P, _P, _L, _P]),
     "LibShalom_sgemm_backward": (_I, [_I, _I, _P, _P, _P, _P, _P, _L, _L, _L, _I, _P]),
+    "LibShalom_sgemm_strided_batched": (_I, [_I, _I, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _I, _P]),
     "LibShalom_sgemm_backward_strided": (_I, [_I, _I, _P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _I, _P]),
     "LibShalom_query_plan": (_I, [_I, _I, _L, _L, _L, _I] + [ct.POINTER(_I)] * 4),
     "LibShalom_query_batched_tile": (_I, [_I, _L, _L, _L, ct.POINTER(_I), ct.POINTER(_I)]),
@@ -209,6 +210,15 @@ def sgemm_backward(transa, transb, dC, A, B, dA, dB, M, N, K, precision=FP32, st
     """dA = dC op(B)^T and dB = op(A)^T dC (each stored like its operand), one launch."""
     _check(_lib.LibShalom_sgemm_backward(transa, transb, _ptr(dC), _ptr(A), _ptr(B), _ptr(dA), _ptr(dB),
                                          M, N, K, precision, _stream(stream)), "LibShalom_sgemm_backward")
+
+
+def sgemm_strided_batched(transa, transb, C, A, B, M, N, K, batch, strideA, strideB, strideC, precision=FP32,
+                          stream=None):
+    """C_i = op(A_i) op(B_i) for a strided batch, one launch; stride 0 shares an operand (or, for C,
+    sums over the batch)."""
+    _check(_lib.LibShalom_sgemm_strided_batched(transa, transb, _ptr(C), _ptr(A), _ptr(B), M, N, K, batch,
+                                                strideA, strideB, strideC, precision, _stream(stream)),
+           "LibShalom_sgemm_strided_batched")
 
 
 def sgemm_backward_strided(transa, transb, dC, A, B, dA, dB, M, N, K, batch, strideA, strideB, strideC,

@@ -361,6 +361,71 @@ void bwd_slice(BwdProblem &P, int T, long len)
     P.units = (int)(outs * P.ntiles * P.nslices);
 }
 
+// Tile size and reduction slices for problems that share one launch.
+// 128 x 128 tiles when every output dimension fills them and they alone give
+// every compute unit a unit of work, else 64 x 64.  A reduction is cut into
+// slices of about W / (2 * CUs) (W = all tiles' reduction work) so that no
+// unit is much longer than a compute unit's share; slices cost partial-tile
+// traffic, which is kept under a quarter of the operand traffic.  Returns
+// the tile size (0: nothing to launch, < 0: -status).
+int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu, hipStream_t s)
+{
+    auto base_units = [&](int T) {
+        long u = 0;
+        for (int i = 0; i < n; ++i) {
+            const BwdProblem &P = *probs[i];
+            u += (P.sO == 0 ? 1L : P.nimg) * ceil_div(P.R, T) * ceil_div(P.Cc, T);
+        }
+        return u;
+    };
+    const int T = (allow_big && base_units(128) >= num_cu) ? 128 : 64;
+    double W = 0, ops = 0;
+    for (int i = 0; i < n; ++i) {
+        const BwdProblem &P = *probs[i];
+        const double outs = P.sO == 0 ? 1 : P.nimg;
+        const double total = (P.sO == 0 ? (double)P.nimg : 1.0) * std::max(P.red, 1);
+        W += outs * ceil_div(P.R, T) * ceil_div(P.Cc, T) * total;
+        ops += 4.0 * ((P.sL == 0 ? 1.0 : P.nimg) * P.R * P.red + (P.sR == 0 ? 1.0 : P.nimg) * P.red * P.Cc +
+                      outs * P.R * P.Cc);
+    }
+    long len = std::max(256L, (long)std::ceil(W / (2.0 * num_cu)));
+    len = ceil_div(len, 32) * 32;
+    size_t slot_bytes = 0, counters = 0;
+    long units = 0;
+    for (int it = 0; it < 40; ++it) {
+        slot_bytes = counters = 0;
+        units = 0;
+        for (int i = 0; i < n; ++i) {
+            BwdProblem &P = *probs[i];
+            bwd_slice(P, T, len);
+            units += P.units;
+            if (P.nslices > 1) {
+                slot_bytes += (size_t)P.units * T * T * sizeof(float);
+                counters += (size_t)(P.sO == 0 ? 1 : P.nimg) * P.ntiles;
+            }
+        }
+        const bool traffic_ok = 2.0 * slot_bytes <= 0.25 * ops;
+        if (traffic_ok && slot_bytes <= ((size_t)256 << 20) && counters * sizeof(int) <= kCounterBytes) break;
+        len *= 2;
+    }
+    if (units > INT32_MAX) return -LIBSHALOM_NOT_SUPPORTED;
+    if (units == 0) return 0;
+    if (counters == 0) return T;
+    int *cnt;
+    float *slots;
+    int st = fixup_space(dev, s, slot_bytes, &cnt, &slots);
+    if (st) return -st;
+    for (int i = 0; i < n; ++i) {
+        BwdProblem &P = *probs[i];
+        if (P.nslices <= 1) continue;
+        P.counters = cnt;
+        P.slots = slots;
+        cnt += (size_t)(P.sO == 0 ? 1 : P.nimg) * P.ntiles;
+        slots += (size_t)P.units * T * T;
+    }
+    return T;
+}
+
 int backward_run(int prec, int ta, int tb, const float *dC, const float *A, const float *B, float *dA, float *dB,
                  long M, long N, long K, int nimg, long sA, long sB, long sC, hipStream_t s)
 {
@@ -379,46 +444,31 @@ int backward_run(int prec, int ta, int tb, const float *dC, const float *A, cons
         bwd_side(Y, A, sA, (int)(ta ? M : K), dC, sC, (int)N, dB, sB, (int)N, K, N, M, nimg);
     else
         bwd_side(Y, dC, sC, (int)N, A, sA, (int)(ta ? M : K), dB, sB, (int)K, N, K, M, nimg);
-    // 128 x 128 tiles when every output dimension fills them, else 64 x 64
-    const int big = std::min(std::min(X.R, X.Cc), std::min(Y.R, Y.Cc)) >= 128;
-    const int T = big ? 128 : 64;
-    // slice length: about four units per compute unit, at least 8 K chunks
-    // per slice, and at most 256 MiB of partial tiles
-    auto work = [&](const BwdProblem &P) {
-        const long outs = (P.sO == 0) ? 1 : P.nimg;
-        const long total = (P.sO == 0) ? (long)P.nimg * P.red : P.red;
-        return (double)outs * ceil_div(P.R, T) * ceil_div(P.Cc, T) * std::max(total, 1L);
-    };
-    const double W = work(X) + work(Y);
-    long len = std::max(256L, (long)std::ceil(W / (4.0 * info.num_cu)));
-    len = ceil_div(len, 32) * 32;
-    size_t slot_bytes = 0;
-    for (int it = 0; it < 32; ++it) {
-        bwd_slice(X, T, len);
-        bwd_slice(Y, T, len);
-        slot_bytes = 0;
-        if (X.nslices > 1) slot_bytes += (size_t)X.units * T * T * sizeof(float);
-        if (Y.nslices > 1) slot_bytes += (size_t)Y.units * T * T * sizeof(float);
-        const size_t counters = (size_t)((X.sO == 0 ? 1 : nimg) * X.ntiles + (Y.sO == 0 ? 1 : nimg) * Y.ntiles);
-        if (slot_bytes <= ((size_t)256 << 20) && counters * sizeof(int) <= kCounterBytes) break;
-        len *= 2;
-    }
-    if (X.units + Y.units == 0) return LIBSHALOM_SUCCESS;
-    if ((long)X.units + Y.units > INT32_MAX) return LIBSHALOM_NOT_SUPPORTED;
-    if (X.nslices > 1 || Y.nslices > 1) {
-        int *counters;
-        float *slots;
-        st = fixup_space(dev, s, slot_bytes, &counters, &slots);
-        if (st) return st;
-        const long xt = X.nslices > 1 ? (long)(X.sO == 0 ? 1 : nimg) * X.ntiles : 0;
-        X.counters = counters;
-        Y.counters = counters + xt;
-        X.slots = slots;
-        Y.slots = slots + (X.nslices > 1 ? (size_t)X.units * T * T : 0);
-    }
-    BwdLaunch fn = bwd_table().fn[prec][ta][tb][big];
+    BwdProblem *probs[2] = {&X, &Y};
+    const int T = plan_problems(probs, 2, std::min(std::min(X.R, X.Cc), std::min(Y.R, Y.Cc)) >= 128, dev,
+                                info.num_cu, s);
+    if (T <= 0) return -T;
+    BwdLaunch fn = bwd_table().fn[prec][ta][tb][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
     return fn(X, Y, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
+}
+
+// C_i = op(A_i) op(B_i) over a strided batch; sC == 0 sums over the batch.
+int strided_run(int prec, int ta, int tb, float *C, const float *A, const float *B, long M, long N, long K, int nimg,
+                long sA, long sB, long sC, hipStream_t s)
+{
+    int dev;
+    DeviceInfo info;
+    int st = current_device(&dev, &info);
+    if (st) return st;
+    BwdProblem P;
+    bwd_side(P, A, sA, (int)(ta ? M : K), B, sB, (int)(tb ? K : N), C, sC, (int)N, M, N, K, nimg);
+    BwdProblem *probs[1] = {&P};
+    const int T = plan_problems(probs, 1, std::min(M, N) >= 128, dev, info.num_cu, s);
+    if (T <= 0) return -T;
+    StridedLaunch fn = bwd_table().strided[prec][ta][tb][T == 128];
+    if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    return fn(P, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }
 
 int sgemm_sync(int ta, int tb, float *C, float *A, float *B, long M, long N, long K, bool mp)
@@ -790,6 +840,21 @@ int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, co
         return set_status(LIBSHALOM_INVALID_POINTER);
     return set_status(backward_run(kprec(precision), transa, transb, dC, A, B, dA, dB, M, N, K, (int)batch,
                                    strideA, strideB, strideC, (hipStream_t)stream));
+}
+
+int LibShalom_sgemm_strided_batched(int transa, int transb, float *C, const float *A, const float *B, long M,
+                                    long N, long K, long batch, long strideA, long strideB, long strideC,
+                                    int precision, LibShalom_stream_t stream)
+{
+    int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
+    if (!st) st = check_dims(transa, transb, M, N, K);
+    if (!st && (batch < 0 || batch > INT32_MAX || strideA < 0 || strideB < 0 || strideC < 0))
+        st = LIBSHALOM_INVALID_VALUE;
+    if (st) return set_status(st);
+    if (batch == 0) return set_status(LIBSHALOM_SUCCESS);
+    if ((M * N > 0 && !C) || (M * N > 0 && K > 0 && (!A || !B))) return set_status(LIBSHALOM_INVALID_POINTER);
+    return set_status(strided_run(kprec(precision), transa, transb, C, A, B, M, N, K, (int)batch, strideA, strideB,
+                                  strideC, (hipStream_t)stream));
 }
 
 int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_cu, int *tile_m, int *tile_n,

@@ -173,10 +173,32 @@ k_gemm_backward(BwdProblem X, BwdProblem Y)
         bwd_unit<CY>(Y, u - X.units, lds, &flag);
 }
 
+// One problem on its own: a strided batch of GEMMs C_i = op(A_i) op(B_i)
+// (a stride of 0 shares an operand; a C stride of 0 sums over the batch).
+template <class Cfg>
+__global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+                          amdgpu_waves_per_eu(Cfg::BM * Cfg::BN >= 128 * 128 ? 2 : 1, 8))) void
+k_gemm_strided(BwdProblem P)
+{
+    __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
+    __shared__ int flag;
+    bwd_unit<Cfg>(P, blockIdx.x, lds, &flag);
+}
+
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
 typedef hipError_t (*BwdLaunch)(const BwdProblem &X, const BwdProblem &Y, hipStream_t s);
+typedef hipError_t (*StridedLaunch)(const BwdProblem &P, hipStream_t s);
+
+template <int PREC, int TA, int TB, int BT>
+hipError_t launch_strided(const BwdProblem &P, hipStream_t s)
+{
+    typedef TileCfg<PREC, TA, TB, BT, BT> Cfg;
+    if (P.units <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gemm_strided<Cfg>, dim3(P.units), dim3(kThreads), 0, s, P);
+    return hipGetLastError();
+}
 
 // (transa, transb) of the forward product -> (TA, TB) of the dA and dB
 // problems (left operand transposed / right operand transposed).
@@ -197,6 +219,7 @@ hipError_t launch_backward(const BwdProblem &X, const BwdProblem &Y, hipStream_t
 constexpr int NBWD_TILE = 2;  // 0: 64 x 64, 1: 128 x 128
 struct BwdTable {
     BwdLaunch fn[NPREC][2][2][NBWD_TILE];
+    StridedLaunch strided[NPREC][2][2][NBWD_TILE];
 };
 BwdTable &bwd_table();
 
@@ -211,6 +234,14 @@ void register_backward(BwdTable &t)
     t.fn[PREC][0][1][1] = &launch_backward<PREC, 0, 1, 128>;
     t.fn[PREC][1][0][1] = &launch_backward<PREC, 1, 0, 128>;
     t.fn[PREC][1][1][1] = &launch_backward<PREC, 1, 1, 128>;
+    t.strided[PREC][0][0][0] = &launch_strided<PREC, 0, 0, 64>;
+    t.strided[PREC][0][1][0] = &launch_strided<PREC, 0, 1, 64>;
+    t.strided[PREC][1][0][0] = &launch_strided<PREC, 1, 0, 64>;
+    t.strided[PREC][1][1][0] = &launch_strided<PREC, 1, 1, 64>;
+    t.strided[PREC][0][0][1] = &launch_strided<PREC, 0, 0, 128>;
+    t.strided[PREC][0][1][1] = &launch_strided<PREC, 0, 1, 128>;
+    t.strided[PREC][1][0][1] = &launch_strided<PREC, 1, 0, 128>;
+    t.strided[PREC][1][1][1] = &launch_strided<PREC, 1, 1, 128>;
 }
 
 void register_backward_fp32(BwdTable &t);

@@ -206,3 +206,40 @@ def test_backward_workspace_failure(dev):
     finally:
         del os.environ["LIBSHALOM_FAIL_WORKSPACE"]
     torch.cuda.synchronize()
+
+
+def strided_ref(ta, tb, A, B):
+    A64, B64 = A.double(), B.double()
+    opA = A64.transpose(-1, -2).contiguous() if ta else A64
+    opB = B64.transpose(-1, -2).contiguous() if tb else B64
+    return opA @ opB, opA.abs() @ opB.abs()
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_strided_batched_forward(dev, ta, tb, prec):
+    g = torch.Generator(device="cpu").manual_seed(17)
+    for (M, N, K, batch, shareA, shareB) in [(70, 90, 300, 5, False, False), (64, 3136, 576, 3, True, False),
+                                            (512, 49, 4608, 2, True, False), (33, 17, 5000, 4, False, True),
+                                            (1, 1, 1, 3, False, False), (200, 150, 130, 2, False, False)]:
+        print(f"strided ta={ta} tb={tb} prec={prec} {M}x{N}x{K} batch {batch}", flush=True)
+        A = torch.randn(*((1,) if shareA else (batch,)), *oracle.a_shape(ta, M, K), generator=g).to(dev)
+        B = torch.randn(*((1,) if shareB else (batch,)), *oracle.b_shape(tb, N, K), generator=g).to(dev)
+        C = torch.full((batch, M, N), float("nan"), device=dev)
+        ls.sgemm_strided_batched(ta, tb, C, A, B, M, N, K, batch, 0 if shareA else M * K, 0 if shareB else K * N,
+                                 M * N, prec)
+        torch.cuda.synchronize()
+        ref, absp = strided_ref(ta, tb, A.expand(batch, *A.shape[1:]), B.expand(batch, *B.shape[1:]))
+        assert_within(C, ref, absp, K, prec, f"C {M}x{N}x{K}")
+
+
+def test_strided_batched_sum_over_batch(dev):
+    # strideC = 0: C = sum_i A_i B_i (one output, reduction over images and K)
+    batch, M, N, K = 7, 96, 80, 200
+    A = torch.randn(batch, M, K, device=dev)
+    B = torch.randn(batch, K, N, device=dev)
+    C = torch.full((M, N), float("nan"), device=dev)
+    ls.sgemm_strided_batched(0, 0, C, A, B, M, N, K, batch, M * K, K * N, 0)
+    torch.cuda.synchronize()
+    ref, absp = strided_ref(0, 0, A, B)
+    assert_within(C, ref.sum(0), absp.sum(0), batch * K, ls.FP32, "sum")
