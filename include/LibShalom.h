@@ -156,6 +156,24 @@ int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, co
                                      long batch, long strideA, long strideB, long strideC,
                                      int precision, LibShalom_stream_t stream);
 
+/* Autotuner for batched plans (no reference counterpart).  It runs the given
+ * batch of shapes on the device (synthetic operands in a scratch buffer; the
+ * caller's memory is not touched), groups the GEMMs by max(M, N) and K
+ * ranges, and for each group in order of work tries every other candidate
+ * square tile (32, 64, 128; FP64: 32, 64), keeping it when the whole batch
+ * runs faster.  The chosen tile is recorded per (precision, transa, transb,
+ * M/N/K class); plans built afterwards use it.  Synchronous on `stream`;
+ * `classes_tuned` receives the number of classes recorded.
+ * LIBSHALOM_TUNE_FILE names a text file the table is read from at first use
+ * and appended to.  query_tuned_tile reports the tile a plan would use
+ * (recorded, or the fixed rule); autotune_reset forgets the table. */
+int LibShalom_autotune_batched(int precision, int transa, int transb, const long *M, const long *N,
+                               const long *K, long count, int *classes_tuned,
+                               LibShalom_stream_t stream);
+int LibShalom_autotune_reset(void);
+int LibShalom_query_tuned_tile(int precision, int transa, int transb, long M, long N, long K,
+                               int *tile_m, int *tile_n);
+
 /* Planner introspection (host only, no device needed): the workgroup tile
  * (tile_m x tile_n of C), K-split count and K slice length the library would
  * use for one GEMM on a device with num_cu compute units (mp != 0: the

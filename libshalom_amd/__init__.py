@@ -21,7 +21,8 @@ __all__ = [
     "LibShalomError", "lib", "sgemm", "sgemm_mp", "dgemm", "sgemm_ex", "sgemm_mp_ex",
     "dgemm_ex", "set_thread_nums", "set_thread_ways", "get_thread_config",
     "set_precision", "get_precision", "BatchedPlan", "sgemm_batched", "dgemm_batched",
-    "sgemm_backward", "sgemm_backward_strided", "sgemm_strided_batched", "query_plan", "query_batched_tile", "LIB_PATH",
+    "sgemm_backward", "sgemm_backward_strided", "sgemm_strided_batched", "autotune_batched",
+    "autotune_reset", "query_tuned_tile", "query_plan", "query_batched_tile", "LIB_PATH",
 ]
 
 NoTrans = 0
@@ -82,6 +83,9 @@ SIGNATURES = {
     "LibShalom_sgemm_backward": (_I, [_I, _I, _P, _P, _P, _P, _P, _L, _L, _L, _I, _P]),
     "LibShalom_sgemm_strided_batched": (_I, [_I, _I, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _I, _P]),
     "LibShalom_sgemm_backward_strided": (_I, [_I, _I, _P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _I, _P]),
+    "LibShalom_autotune_batched": (_I, [_I, _I, _I, _P, _P, _P, _L, ct.POINTER(_I), _P]),
+    "LibShalom_autotune_reset": (_I, []),
+    "LibShalom_query_tuned_tile": (_I, [_I, _I, _I, _L, _L, _L, ct.POINTER(_I), ct.POINTER(_I)]),
     "LibShalom_query_plan": (_I, [_I, _I, _L, _L, _L, _I] + [ct.POINTER(_I)] * 4),
     "LibShalom_query_batched_tile": (_I, [_I, _L, _L, _L, ct.POINTER(_I), ct.POINTER(_I)]),
 }
@@ -293,6 +297,33 @@ def dgemm_batched(transa, transb, Cs, As, Bs, Ms, Ns, Ks, stream=None):
     n, cp, ap, bp, mp, np_, kp = _arrays(Cs, As, Bs, Ms, Ns, Ks)
     _check(_lib.LibShalom_dgemm_batched(transa, transb, cp, ap, bp, mp, np_, kp, n, _stream(stream)),
            "LibShalom_dgemm_batched")
+
+
+def autotune_batched(precision, transa, transb, Ms, Ns, Ks, stream=None):
+    """Time the candidate tiles per shape class present in (Ms, Ns, Ks) and record the
+    fastest for later batched plans; returns the number of classes tuned."""
+    import numpy as np
+    m = np.ascontiguousarray(Ms, dtype=np.int64)
+    n = np.ascontiguousarray(Ns, dtype=np.int64)
+    k = np.ascontiguousarray(Ks, dtype=np.int64)
+    if not (len(m) == len(n) == len(k)):
+        raise ValueError("Ms, Ns, Ks differ in length")
+    tuned = _I(0)
+    _check(_lib.LibShalom_autotune_batched(precision, transa, transb, m.ctypes.data, n.ctypes.data,
+                                           k.ctypes.data, len(m), ct.byref(tuned), _stream(stream)),
+           "LibShalom_autotune_batched")
+    return tuned.value
+
+
+def autotune_reset():
+    _check(_lib.LibShalom_autotune_reset(), "LibShalom_autotune_reset")
+
+
+def query_tuned_tile(precision, transa, transb, M, N, K):
+    tm, tn = _I(0), _I(0)
+    _check(_lib.LibShalom_query_tuned_tile(precision, transa, transb, M, N, K, ct.byref(tm), ct.byref(tn)),
+           "LibShalom_query_tuned_tile")
+    return tm.value, tn.value
 
 
 def query_plan(precision, M, N, K, num_cu=256, mp=False):

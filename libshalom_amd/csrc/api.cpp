@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -538,6 +539,265 @@ struct Bucket {
     size_t off = 0;  // byte offset of the bucket's jobs in the plan buffer
 };
 
+// ---------------------------------------------------------------------------
+// Autotuner for the batched tile buckets.  The table maps (precision,
+// transa, transb, shape class) to a tile; shape classes are ranges of M, N
+// and K.  Tuning works on a whole batch: the GEMMs are grouped by
+// (max(M, N) range, K range), and, group by group in order of work, each
+// other candidate square tile (32, 64, 128; FP64: 32, 64) is tried for the
+// group while the rest keep their current tiles; a candidate is kept when
+// the batch -- every bucket launched as a plan would launch it, on synthetic
+// operands in a scratch buffer -- runs at least 1% faster.  Timing the whole
+// batch keeps what the buckets share (the chip, L2, the tail of each launch)
+// in the measurement.  Plans built afterwards use the table; classes never
+// tuned keep the fixed rule.  LIBSHALOM_TUNE_FILE names a text file the
+// table is read from at first use and appended to after each tuning run.
+// ---------------------------------------------------------------------------
+constexpr int kDimEdges[] = {16, 32, 48, 64, 96, 128};  // M and N classes: <= edge, then > 128
+constexpr int kKEdges[] = {16, 32, 64, 128};            // K classes
+int dim_class(long v, const int *edges, int n)
+{
+    for (int i = 0; i < n; ++i)
+        if (v <= edges[i]) return i;
+    return n;
+}
+int shape_class(long M, long N, long K)
+{
+    const int nd = 7, nk = 5;
+    return (dim_class(M, kDimEdges, 6) * nd + dim_class(N, kDimEdges, 6)) * nk + dim_class(K, kKEdges, 4);
+}
+int tune_group(long M, long N, long K) { return dim_class(std::max(M, N), kDimEdges, 6) * 5 + dim_class(K, kKEdges, 4); }
+long tune_key(int prec, int ta, int tb, int cls) { return (((long)prec * 2 + ta) * 2 + tb) * 1000 + cls; }
+
+std::mutex g_tune_mu;
+std::map<long, int> g_tune;  // key -> TileId
+bool g_tune_loaded = false;
+
+void tune_load_locked()
+{
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    const char *path = std::getenv("LIBSHALOM_TUNE_FILE");
+    if (!path || !path[0]) return;
+    FILE *f = std::fopen(path, "r");
+    if (!f) return;
+    int prec, ta, tb, cls, tile;
+    while (std::fscanf(f, "%d %d %d %d %d", &prec, &ta, &tb, &cls, &tile) == 5)
+        if (prec >= 0 && prec < NPREC && (ta | tb) >= 0 && ta < 2 && tb < 2 && tile >= 0 && tile < NTILE)
+            g_tune[tune_key(prec, ta, tb, cls)] = tile;
+    std::fclose(f);
+}
+
+int tuned_tile(int prec, int ta, int tb, long M, long N, long K)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
+        auto it = g_tune.find(tune_key(prec, ta, tb, shape_class(M, N, K)));
+        if (it != g_tune.end()) return it->second;
+    }
+    return batched_tile(prec, M, N, K);
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// Time the batch `descs` with tile assign[i] for GEMM i, buckets launched as
+// a plan launches them (ms per execution, best of `reps`).
+int time_assignment(int prec, int ta, int tb, const std::vector<GemmDesc> &descs, const std::vector<int> &assign,
+                    int esize, int num_cu, hipStream_t s, DevBuf &jobs_dev, size_t &jobs_cap, hipEvent_t e0,
+                    hipEvent_t e1, float *ms_out)
+{
+    std::vector<std::vector<GemmDesc>> b(NTILE);
+    for (size_t i = 0; i < descs.size(); ++i) {
+        const GemmDesc &d = descs[i];
+        const TileShape ts = kTileShapes[assign[i]];
+        for (int m0 = 0; m0 < d.M; m0 += ts.bm)
+            for (int n0 = 0; n0 < d.N; n0 += ts.bn)
+                b[assign[i]].push_back(tile_job(ta, tb, d, m0, n0, ts.bm, ts.bn, esize));
+    }
+    size_t total = 0;
+    std::vector<size_t> off(NTILE, 0);
+    for (int t = 0; t < NTILE; ++t) {
+        std::stable_sort(b[t].begin(), b[t].end(),
+                         [](const GemmDesc &x, const GemmDesc &y) { return job_cost(x) > job_cost(y); });
+        off[t] = total;
+        total += ((b[t].size() * sizeof(GemmDesc)) + 255) & ~(size_t)255;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    if (total > jobs_cap) {
+        if (jobs_dev.p) (void)hipFree(jobs_dev.p);
+        jobs_dev.p = nullptr;
+        if (hipMalloc(&jobs_dev.p, total) != hipSuccess) {
+            (void)hipGetLastError();
+            jobs_cap = 0;
+            return LIBSHALOM_ALLOC_FAILED;
+        }
+        jobs_cap = total;
+    }
+    std::vector<char> host(total);
+    for (int t = 0; t < NTILE; ++t)
+        if (!b[t].empty()) std::memcpy(host.data() + off[t], b[t].data(), b[t].size() * sizeof(GemmDesc));
+    if (total && hipMemcpy(jobs_dev.p, host.data(), total, hipMemcpyHostToDevice) != hipSuccess)
+        return LIBSHALOM_HIP_ERROR;
+    auto run = [&]() {
+        const int order[NTILE] = {T128x128, T128x64, T128x32, T32x128, T64x64, T32x32};
+        for (int oi = 0; oi < NTILE; ++oi) {
+            const int t = order[oi];
+            if (b[t].empty()) continue;
+            PersistLaunch fn = launch_table().persist[prec][ta][tb][t];
+            PersistOccupancy occ = launch_table().persist_occ[prec][ta][tb][t];
+            if (!fn) return false;
+            const long nj = (long)b[t].size();
+            const int grid = (int)std::min<long>(nj, (long)std::max(1, occ ? occ() : 1) * num_cu);
+            if (fn((const GemmDesc *)((char *)jobs_dev.p + off[t]), (int)nj, grid, s) != hipSuccess) return false;
+        }
+        return true;
+    };
+    if (!run()) return LIBSHALOM_HIP_ERROR;  // warm-up
+    float best = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+        if (hipEventRecord(e0, s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+        if (!run()) return LIBSHALOM_HIP_ERROR;
+        if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+        best = std::min(best, ms);
+    }
+    *ms_out = best;
+    return LIBSHALOM_SUCCESS;
+}
+
+int autotune(int prec, int ta, int tb, const long *M, const long *N, const long *K, long count, hipStream_t s,
+             int *classes_tuned)
+{
+    int dev;
+    DeviceInfo info;
+    int st = current_device(&dev, &info);
+    if (st) return st;
+    const bool f64 = prec == P_FP64;
+    const int esize = f64 ? 8 : 4;
+    // the batch (at most 64 K GEMMs and 4 GiB of operands: an even sample of larger ones)
+    size_t bytes_all = 0;
+    long nonempty = 0;
+    for (long g = 0; g < count; ++g) {
+        st = check_dims(ta, tb, M[g], N[g], K[g]);
+        if (st) return st;
+        if (M[g] == 0 || N[g] == 0) continue;
+        bytes_all += (size_t)esize * (M[g] * K[g] + K[g] * N[g] + M[g] * N[g]) + 64;
+        ++nonempty;
+    }
+    if (nonempty == 0) return LIBSHALOM_SUCCESS;
+    const double keep = std::min(1.0, std::min(65536.0 / nonempty, (double)((size_t)4 << 30) / bytes_all));
+    std::vector<long> sel;
+    double acc = 0;
+    for (long g = 0; g < count; ++g) {
+        if (M[g] == 0 || N[g] == 0) continue;
+        acc += keep;
+        if (acc >= 1.0) {
+            acc -= 1.0;
+            sel.push_back(g);
+        }
+    }
+    size_t need = 0;
+    for (long g : sel) need += (size_t)esize * (M[g] * K[g] + K[g] * N[g] + M[g] * N[g]) + 64;
+    DevBuf scratch, jobs_dev;
+    if (hipMalloc(&scratch.p, need) != hipSuccess) {
+        (void)hipGetLastError();
+        return LIBSHALOM_ALLOC_FAILED;
+    }
+    if (hipMemsetAsync(scratch.p, 0, need, s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    std::vector<GemmDesc> descs;
+    std::vector<int> group, assign;
+    char *base = (char *)scratch.p;
+    for (long g : sel) {
+        const void *A = base;
+        base += (size_t)esize * M[g] * K[g];
+        const void *B = base;
+        base += (size_t)esize * K[g] * N[g];
+        void *C = base;
+        base += (size_t)esize * M[g] * N[g] + 64;
+        descs.push_back(make_desc(ta, tb, A, B, C, M[g], N[g], K[g], esize));
+        group.push_back(tune_group(M[g], N[g], K[g]));
+        assign.push_back(tuned_tile(prec, ta, tb, M[g], N[g], K[g]));
+    }
+    // groups by work
+    std::map<int, double> gwork;
+    for (size_t i = 0; i < descs.size(); ++i) gwork[group[i]] += job_cost(descs[i]);
+    std::vector<std::pair<double, int>> order;
+    for (auto &kv : gwork) order.push_back({-kv.second, kv.first});
+    std::sort(order.begin(), order.end());
+
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return LIBSHALOM_HIP_ERROR;
+    }
+    size_t jobs_cap = 0;
+    float best = 0;
+    st = time_assignment(prec, ta, tb, descs, assign, esize, info.num_cu, s, jobs_dev, jobs_cap, e0, e1, &best);
+    const int cands[3] = {T32x32, T64x64, T128x128};
+    const int ncand = f64 ? 2 : 3;
+    for (size_t oi = 0; oi < order.size() && !st; ++oi) {
+        const int g = order[oi].second;
+        for (int ci = 0; ci < ncand && !st; ++ci) {
+            std::vector<int> trial = assign;
+            bool changed = false;
+            for (size_t i = 0; i < descs.size(); ++i)
+                if (group[i] == g && trial[i] != cands[ci]) {
+                    trial[i] = cands[ci];
+                    changed = true;
+                }
+            if (!changed) continue;
+            float ms = 0;
+            st = time_assignment(prec, ta, tb, descs, trial, esize, info.num_cu, s, jobs_dev, jobs_cap, e0, e1, &ms);
+            if (!st && ms < 0.99f * best) {
+                best = ms;
+                assign.swap(trial);
+            }
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (st) return st;
+    if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    // record the tile of every shape class seen (all GEMMs of a group share it)
+    std::map<long, int> found;
+    for (size_t i = 0; i < descs.size(); ++i)
+        found[tune_key(prec, ta, tb, shape_class(descs[i].M, descs[i].N, descs[i].K))] = assign[i];
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        tune_load_locked();
+        for (auto &f : found) g_tune[f.first] = f.second;
+        const char *path = std::getenv("LIBSHALOM_TUNE_FILE");
+        if (path && path[0] && !found.empty()) {
+            if (FILE *fp = std::fopen(path, "a")) {
+                for (auto &f : found) {
+                    const long key = f.first;
+                    const int cls = (int)(key % 1000), rest = (int)(key / 1000);
+                    std::fprintf(fp, "%d %d %d %d %d\n", rest / 4, (rest / 2) % 2, rest % 2, cls, f.second);
+                }
+                std::fclose(fp);
+            }
+        }
+    }
+    if (classes_tuned) *classes_tuned = (int)found.size();
+    return LIBSHALOM_SUCCESS;
+}
+
+void tune_reset()
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.clear();
+    g_tune_loaded = true;  // a reset table stays empty (no reload from the file)
+}
+
 int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, const void *const *A,
                const void *const *B, const long *M, const long *N, const long *K, long count)
 {
@@ -566,7 +826,7 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
         if (M[g] == 0 || N[g] == 0) continue;
         flops += 2.0 * M[g] * N[g] * K[g];
         bytes += (double)esize * (M[g] * K[g] + K[g] * N[g] + M[g] * N[g]);
-        const int tile = mode == BM_SINGLE ? single_tile : batched_tile(prec, M[g], N[g], K[g]);
+        const int tile = mode == BM_SINGLE ? single_tile : tuned_tile(prec, ta, tb, M[g], N[g], K[g]);
         const GemmDesc d = make_desc(ta, tb, A[g], B[g], C[g], M[g], N[g], K[g], esize);
         Bucket &b = buckets[tile];
         const TileShape ts = kTileShapes[tile];
@@ -855,6 +1115,36 @@ int LibShalom_sgemm_strided_batched(int transa, int transb, float *C, const floa
     if ((M * N > 0 && !C) || (M * N > 0 && K > 0 && (!A || !B))) return set_status(LIBSHALOM_INVALID_POINTER);
     return set_status(strided_run(kprec(precision), transa, transb, C, A, B, M, N, K, (int)batch, strideA, strideB,
                                   strideC, (hipStream_t)stream));
+}
+
+int LibShalom_autotune_batched(int precision, int transa, int transb, const long *M, const long *N, const long *K,
+                               long count, int *classes_tuned, LibShalom_stream_t stream)
+{
+    if (!valid_precision(precision) && precision != LIBSHALOM_FP64) return set_status(LIBSHALOM_INVALID_VALUE);
+    if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1) || count < 0)
+        return set_status(LIBSHALOM_INVALID_VALUE);
+    if (count > 0 && (!M || !N || !K)) return set_status(LIBSHALOM_INVALID_POINTER);
+    if (classes_tuned) *classes_tuned = 0;
+    return set_status(autotune(kprec(precision), transa, transb, M, N, K, count, (hipStream_t)stream,
+                               classes_tuned));
+}
+
+int LibShalom_autotune_reset(void)
+{
+    tune_reset();
+    return set_status(LIBSHALOM_SUCCESS);
+}
+
+int LibShalom_query_tuned_tile(int precision, int transa, int transb, long M, long N, long K, int *tile_m,
+                               int *tile_n)
+{
+    if (!valid_precision(precision) && precision != LIBSHALOM_FP64) return set_status(LIBSHALOM_INVALID_VALUE);
+    if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1) || M < 0 || N < 0 || K < 0)
+        return set_status(LIBSHALOM_INVALID_VALUE);
+    const int t = tuned_tile(kprec(precision), transa, transb, M, N, K);
+    if (tile_m) *tile_m = kTileShapes[t].bm;
+    if (tile_n) *tile_n = kTileShapes[t].bn;
+    return set_status(LIBSHALOM_SUCCESS);
 }
 
 int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_cu, int *tile_m, int *tile_n,

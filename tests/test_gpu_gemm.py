@@ -300,3 +300,22 @@ def test_backward_matches_matmul(dev, ta, tb):
         bndB = (bndB.T if tb else bndB) * 2 * M * U32
         assert ((dA.double() - refA).abs() <= bndA).all()
         assert ((dB.double() - refB).abs() <= bndB).all()
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+def test_autotuned_plan(dev, prec):
+    """Tune the tile per shape class on an uneven batch, then build the plan:
+    every GEMM still meets the bound, and the tiles used are candidates."""
+    from libshalom_amd.workloads import uniform_shapes
+    shapes = uniform_shapes(3000, 1, 128, seed=5)
+    Ms, Ns, Ks = (np.array(x, dtype=np.int64) for x in zip(*shapes))
+    try:
+        n = ls.autotune_batched(prec, 0, 0, Ms, Ns, Ks)
+        assert n > 0
+        tiles = {ls.query_tuned_tile(prec, 0, 0, *s) for s in shapes}
+        assert tiles <= {(32, 32), (64, 64), (128, 128)}
+        outs, Cref, S, Ke, offC = _run_batched(dev, shapes[:600], 0, 0, prec, seed=9)
+        bad, st = oracle.check_kv(outs[0][:-1], Cref, S, Ke, U32, PREC_EXTRA[prec])
+        assert bad == 0, st
+    finally:
+        ls.autotune_reset()

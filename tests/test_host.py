@@ -2,6 +2,7 @@
 what include/LibShalom.h declares, rejects bad arguments before touching the
 device, and its planner obeys its invariants."""
 import ctypes as ct
+import os
 import re
 
 import pytest
@@ -210,3 +211,36 @@ def test_batched_tile_rule(prec):
     assert ls.query_batched_tile(prec, 33, 10, 1) == (64, 64)
     big = ls.query_batched_tile(prec, 100, 128, 7)
     assert big == ((64, 64) if prec == ls.FP64 else (128, 128))
+
+
+def test_tuned_tile_defaults_to_fixed_rule():
+    ls.autotune_reset()
+    for prec in (ls.FP32, ls.BF16X3, ls.FP64):
+        for (M, N, K) in [(1, 1, 1), (17, 40, 9), (64, 64, 64), (65, 3, 128), (128, 128, 1), (300, 20, 77)]:
+            assert ls.query_tuned_tile(prec, 0, 0, M, N, K) == ls.query_batched_tile(prec, M, N, K)
+
+
+def test_autotune_argument_errors_before_device():
+    L = ls.lib()
+    assert L.LibShalom_autotune_batched(9, 0, 0, None, None, None, 0, None, None) == 1
+    assert L.LibShalom_autotune_batched(0, 2, 0, None, None, None, 0, None, None) == 1
+    assert L.LibShalom_autotune_batched(0, 0, 0, None, None, None, 5, None, None) == 2
+    with pytest.raises(ls.LibShalomError):
+        ls.query_tuned_tile(ls.FP32, 0, 0, -1, 4, 4)
+
+
+def test_tune_file_is_loaded(tmp_path):
+    """A table entry in LIBSHALOM_TUNE_FILE (precision transa transb class
+    tile) overrides the fixed rule for that class, in a fresh process."""
+    import subprocess
+    import sys
+    # class of (M, N, K) = (20, 20, 20): M, N in (16, 32], K in (16, 32]
+    cls = (1 * 7 + 1) * 5 + 1
+    path = tmp_path / "tune.txt"
+    path.write_text(f"{ls.BF16X3} 0 1 {cls} 2\n")  # TileId 2 = 128 x 128
+    code = ("import libshalom_amd as ls; print(ls.query_tuned_tile(ls.BF16X3, 0, 1, 20, 20, 20), "
+            "ls.query_tuned_tile(ls.BF16X3, 0, 0, 20, 20, 20))")
+    env = dict(os.environ, LIBSHALOM_TUNE_FILE=str(path))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True,
+                         cwd=os.path.dirname(os.path.dirname(ls.LIB_PATH)))
+    assert out.stdout.split() == ["(128,", "128)", "(32,", "32)"]
