@@ -319,3 +319,21 @@ def test_autotuned_plan(dev, prec):
         assert bad == 0, st
     finally:
         ls.autotune_reset()
+
+
+@pytest.mark.parametrize("prec", [ls.BF16X3, ls.BF16])
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_batched_large_tiles_all_trans(dev, prec, ta, tb):
+    """Batched GEMMs with max(M, N) > 64 and M, N, K <= 128 (the 128 x 128
+    bucket) in the BF16 modes and all four trans combinations, incl. K = 1,
+    K = 128 and N or M = 1, back to back in memory: every element within the
+    bound, replays identical, nothing written past a block."""
+    rng = np.random.default_rng(40 + 2 * ta + tb + 7 * prec)
+    shapes = [tuple(int(x) for x in rng.integers(1, 129, size=3)) for _ in range(300)]
+    shapes += [(128, 128, 128), (128, 1, 128), (1, 128, 65), (65, 70, 1), (100, 128, 31), (97, 3, 128)]
+    shapes = [s for s in shapes if max(s[0], s[1]) > 64]
+    outs, Cref, S, Ke, offC = _run_batched(dev, shapes, ta, tb, prec, seed=77, replays=2)
+    assert np.array_equal(outs[0], outs[1], equal_nan=True)
+    bad, st = oracle.check_kv(outs[0][:-1], Cref, S, Ke, U32, PREC_EXTRA[prec])
+    assert bad == 0, st
+    assert np.isnan(outs[0][-1])
