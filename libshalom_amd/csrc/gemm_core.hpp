@@ -20,9 +20,9 @@
 //   16x16 MFMA sub-tiles entirely outside C are skipped, and for FP32/FP64
 //   the K chunk's LDS layout is permuted so the MFMA k-steps past K are
 //   skipped too.
-// * Register tiling.  Each wave owns a rectangle of 16x16 sub-tiles (at most
-//   BM/32 x BN/32); the wave grid (2x2, 1x4 or 4x1) is chosen per tile at run
-//   time so small or skinny tiles keep all four waves busy.
+// * Register tiling.  Each wave owns one quadrant of the tile, BM/32 x BN/32
+//   sub-tiles of 16x16, multiplied without per-sub-tile branches; a quadrant
+//   outside the job is skipped as a whole.
 // * Pipelining.  A workgroup runs a stream of jobs (tiles of GEMMs) as one
 //   flat sequence of K chunks: two chunks are in flight in registers while
 //   the current one is multiplied out of LDS, across job boundaries; LDS is
@@ -346,31 +346,20 @@ struct Tile {
     typedef Stage<T, Cfg::NP, BN, BK, Cfg::B_KCONTIG> StageB;
 
     Acc acc[WM][WN];
-    int wr0, wc0, wm, wn;  // this wave's sub-tile rectangle (in 16-row/col units)
+    int wr0, wc0;  // this wave's quadrant (WM x WN sub-tiles of 16x16) in 16-row/col units
+    bool live;     // the quadrant holds part of the job's C tile
 
-    // Pick the wave grid for a tile whose valid extent is mt x nt sub-tiles.
+    // The four waves own the four quadrants of the tile.  Every sub-tile of
+    // a live quadrant is multiplied and stored whether or not it reaches into
+    // the job: rows and columns past the job are zeros in LDS, and their
+    // stores are masked by the buffer range.  Issuing the MFMAs without
+    // per-sub-tile branches lets the LDS reads run ahead of them; a quadrant
+    // wholly outside the job skips its work with one uniform branch.
     __device__ __forceinline__ void layout(int mt, int nt, int wave)
     {
-        // candidates: 2x2, 1x4 (needs mt <= WM), 4x1 (needs nt <= WN)
-        int lm = 2, ln = 2;
-        int pm = (mt + 1) >> 1, pn = (nt + 1) >> 1;
-        int best = pm * pn;
-        if (mt <= WM) {
-            int q = (nt + 3) >> 2;
-            if (mt * q < best) { best = mt * q; lm = 1; ln = 4; pm = mt; pn = q; }
-        }
-        if (nt <= WN) {
-            int q = (mt + 3) >> 2;
-            if (q * nt < best) { best = q * nt; lm = 4; ln = 1; pm = q; pn = nt; }
-        }
-        const int wi = wave / ln, wj = wave % ln;
-        wr0 = wi * pm;
-        wc0 = wj * pn;
-        wm = min(pm, mt - wr0);
-        wn = min(pn, nt - wc0);
-        if (wm < 0) wm = 0;
-        if (wn < 0) wn = 0;
-        (void)lm;
+        wr0 = (wave >> 1) * WM;
+        wc0 = (wave & 1) * WN;
+        live = uniform(wr0 < mt && wc0 < nt) != 0;
     }
 
     __device__ __forceinline__ void zero()
@@ -389,30 +378,27 @@ struct Tile {
     __device__ __forceinline__ void compute(const char *__restrict__ buf, int lane, int ksteps)
     {
         const int r = lane & 15, q = lane >> 4;
+        if (!live) return;
         if constexpr (!Cfg::SPLIT) {
             constexpr int NS = Cfg::NSLOT;
             const T *As = (const T *)buf;
             const T *Bs = (const T *)(buf + Cfg::A_BYTES);
             T af[WM][NS];
 #pragma unroll
-            for (int a = 0; a < WM; ++a)
-                if (a < wm) read_frag<typename Cfg::PA>(As, wr0 + a, r, q, af[a]);
+            for (int a = 0; a < WM; ++a) read_frag<typename Cfg::PA>(As, wr0 + a, r, q, af[a]);
 #pragma unroll
             for (int b = 0; b < WN; ++b) {
-                if (b < wn) {
-                    T bf[NS];
-                    read_frag<typename Cfg::PB>(Bs, wc0 + b, r, q, bf);
+                T bf[NS];
+                read_frag<typename Cfg::PB>(Bs, wc0 + b, r, q, bf);
 #pragma unroll
-                    for (int s = 0; s < NS; ++s) {
-                        if (s < ksteps) {
+                for (int s = 0; s < NS; ++s) {
+                    if (s < ksteps) {
 #pragma unroll
-                            for (int a = 0; a < WM; ++a)
-                                if (a < wm) {
-                                    if constexpr (sizeof(T) == 4)
-                                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][s], bf[s], acc[a][b], 0, 0, 0);
-                                    else
-                                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a][s], bf[s], acc[a][b], 0, 0, 0);
-                                }
+                        for (int a = 0; a < WM; ++a) {
+                            if constexpr (sizeof(T) == 4)
+                                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][s], bf[s], acc[a][b], 0, 0, 0);
+                            else
+                                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a][s], bf[s], acc[a][b], 0, 0, 0);
                         }
                     }
                 }
@@ -424,29 +410,25 @@ struct Tile {
             bf16x8 af[WM][NP];
 #pragma unroll
             for (int a = 0; a < WM; ++a)
-                if (a < wm)
 #pragma unroll
-                    for (int pl = 0; pl < NP; ++pl)
-                        af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + ((wr0 + a) * 16 + r) * LDKH + q * 8);
+                for (int pl = 0; pl < NP; ++pl)
+                    af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + ((wr0 + a) * 16 + r) * LDKH + q * 8);
             // partial products (A piece, B piece), smallest first
             constexpr int NPROD = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
             constexpr int PA[6] = {NP == 3 ? 2 : 1, NP == 3 ? 1 : 0, NP == 3 ? 0 : 0, 1, 0, 0};
             constexpr int PB[6] = {0, NP == 3 ? 1 : 1, NP == 3 ? 2 : 0, 0, 1, 0};
 #pragma unroll
             for (int b = 0; b < WN; ++b) {
-                if (b < wn) {
-                    bf16x8 bfr[NP];
+                bf16x8 bfr[NP];
 #pragma unroll
-                    for (int pl = 0; pl < NP; ++pl)
-                        bfr[pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + ((wc0 + b) * 16 + r) * LDKH + q * 8);
+                for (int pl = 0; pl < NP; ++pl)
+                    bfr[pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + ((wc0 + b) * 16 + r) * LDKH + q * 8);
 #pragma unroll
-                    for (int p = 0; p < NPROD; ++p) {
-                        const int ia = NP == 1 ? 0 : PA[p], ib = NP == 1 ? 0 : PB[p];
+                for (int p = 0; p < NPROD; ++p) {
+                    const int ia = NP == 1 ? 0 : PA[p], ib = NP == 1 ? 0 : PB[p];
 #pragma unroll
-                        for (int a = 0; a < WM; ++a)
-                            if (a < wm)
-                                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][ia], bfr[ib], acc[a][b], 0, 0, 0);
-                    }
+                    for (int a = 0; a < WM; ++a)
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[a][ia], bfr[ib], acc[a][b], 0, 0, 0);
                 }
             }
         }
@@ -517,12 +499,16 @@ struct Tile {
 
     __device__ __forceinline__ static int nchunks(int K) { return max(1, (K + BK - 1) / BK); }
 
-    // Position in the flat stream of K chunks of a workgroup's jobs.
+    // Position in the flat stream of K chunks of a workgroup's jobs.  Only
+    // indices are carried; a job's descriptor is fetched from the source
+    // where it is needed (scalar loads from the job array, or a few scalar
+    // operations for computed sources), which keeps the scalar registers of
+    // the loop few enough that uniform values are not moved to vector
+    // registers.
     struct Cursor {
-        GemmDesc d;  // the job (pointers offset to the tile, K from 0)
-        int job;     // job index, -1 past the end
-        int c;       // chunk within the job
-        int nch;     // chunks of the job
+        int job;  // job index, -1 past the end
+        int c;    // chunk within the job
+        int nch;  // chunks of the job
     };
 
     template <class Src>
@@ -536,9 +522,8 @@ struct Tile {
         const int nj = x.job + stride;
         if (nj < src.count()) {
             x.job = nj;
-            x.d = src.get(nj);
             x.c = 0;
-            x.nch = nchunks(x.d.K);
+            x.nch = nchunks(src.get(nj).K);
         } else {
             x.job = -1;
         }
@@ -552,15 +537,27 @@ struct Tile {
                                          StageB &b1, char *lds, int bi, const Src &src, int stride, int tid,
                                          int lane, int wave)
     {
-        load_chunk(a0, b0, n2.d, n2.c * BK, n2.job >= 0, tid);
-        compute(lds_buf(lds, bi), lane, ksteps(cur.d.K, cur.c * BK));
-        const bool chained = (cur.d.flags & kChain) != 0;
-        if (cur.c + 1 == cur.nch && !chained) store((T *)cur.d.C, cur.d.ldc, cur.d.M, cur.d.N);
-        if (n1.job < 0) return false;
-        // n1 starting a job means cur ends one: a chained job keeps its sums
-        if (n1.c == 0 && !chained) begin(n1.d.M, n1.d.N, 0, 0, wave);
+        load_chunk(a0, b0, src.get(max(n2.job, 0)), n2.c * BK, n2.job >= 0, tid);
+        // keep the loads ahead of the multiply in the schedule
+        __builtin_amdgcn_sched_barrier(0);
+        const GemmDesc dc = src.get(cur.job);
+        compute(lds_buf(lds, bi), lane, ksteps(dc.K, cur.c * BK));
+        const bool chained = (dc.flags & kChain) != 0;
+        // n1's chunk goes to LDS even past the end of the stream (it is then
+        // the zeros of a placeholder load), so that every path has waited for
+        // its loads before the C tile below is written: vector stores and
+        // loads share one counter, and stores issued before that wait would
+        // make it wait for n2's loads as well and drain the prefetch at every
+        // job end.
         if constexpr (Cfg::NBUF == 1) __syncthreads();
         chunk_to_lds(a1, b1, lds_buf(lds, bi + 1), tid);
+        if (cur.c + 1 == cur.nch && !chained) store((T *)dc.C, dc.ldc, dc.M, dc.N);
+        if (n1.job < 0) return false;
+        // n1 starting a job means cur ends one: a chained job keeps its sums
+        if (n1.c == 0 && !chained) {
+            const GemmDesc d1 = src.get(n1.job);
+            begin(d1.M, d1.N, 0, 0, wave);
+        }
         __syncthreads();
         cur = n1;
         n1 = n2;
@@ -581,20 +578,20 @@ struct Tile {
         const int lane = tid & (kWave - 1);
         const int wave = uniform(tid >> 6);
         if (first >= src.count()) return;
+        const GemmDesc d0 = src.get(first);
         Cursor cur;
         cur.job = first;
-        cur.d = src.get(first);
         cur.c = 0;
-        cur.nch = nchunks(cur.d.K);
+        cur.nch = nchunks(d0.K);
         Cursor n1 = cur;
         advance(n1, src, stride);
         Cursor n2 = n1;
         advance(n2, src, stride);
         StageA a0, a1;
         StageB b0, b1;
-        load_chunk(a0, b0, cur.d, 0, true, tid);
-        load_chunk(a1, b1, n1.d, n1.c * BK, n1.job >= 0, tid);
-        begin(cur.d.M, cur.d.N, 0, 0, wave);
+        load_chunk(a0, b0, d0, 0, true, tid);
+        load_chunk(a1, b1, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
+        begin(d0.M, d0.N, 0, 0, wave);
         chunk_to_lds(a0, b0, lds_buf(lds, 0), tid);
         __syncthreads();
         for (;;) {
@@ -627,17 +624,17 @@ struct Tile {
         const int lane = threadIdx.x & (kWave - 1);
         const int r = lane & 15, q = lane >> 4;
         const int rstep = (acc_row(0, 1) - acc_row(0, 0)) * ldo * ES_OUT;  // offset between a lane's rows
+        if (!live) return;
 #pragma unroll
         for (int a = 0; a < WM; ++a)
 #pragma unroll
-            for (int b = 0; b < WN; ++b)
-                if (a < wm && b < wn) {
-                    const int col = (wc0 + b) * 16 + r;
-                    const int row = (wr0 + a) * 16 + acc_row(q, 0);
-                    const int off = (row * ldo + col) * ES_OUT | (col < N ? 0 : kOOB);
+            for (int b = 0; b < WN; ++b) {
+                const int col = (wc0 + b) * 16 + r;
+                const int row = (wr0 + a) * 16 + acc_row(q, 0);
+                const int off = (row * ldo + col) * ES_OUT | (col < N ? 0 : kOOB);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) buf_store1<OutT>(rs, off + i * rstep, (OutT)acc[a][b][i]);
-                }
+                for (int i = 0; i < 4; ++i) buf_store1<OutT>(rs, off + i * rstep, (OutT)acc[a][b][i]);
+            }
     }
 };
 
