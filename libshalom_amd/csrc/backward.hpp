@@ -77,7 +77,7 @@ struct SliceSrc {
 };
 
 // Sum the slots of tile tg in slice order into the output (last arrival).
-template <int BM, int BN>
+template <int BM, int BN, int NT>
 __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_out, int r0, int c0, int *flag)
 {
     __threadfence();  // this workgroup's partial tile is visible device-wide
@@ -92,7 +92,7 @@ __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_o
     const int tr = min(BM, P.R - r0), tcn = min(BN, P.Cc - c0);
     const float *base = P.slots + (size_t)tg * P.nslices * (BM * BN);
     float *O = (float *)P.O + (long long)img_out * P.sO;
-    for (int e = threadIdx.x * 4; e < tr * BN; e += kThreads * 4) {
+    for (int e = threadIdx.x * 4; e < tr * BN; e += NT * 4) {
         const int r = e / BN, c = e % BN;
         if (c >= tcn) continue;
         f32x4 acc = *(const f32x4 *)(base + e);
@@ -151,7 +151,7 @@ __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, 
     }
     Tile<Cfg> tile;
     tile.stream(src, 0, 1, lds);
-    if (P.nslices > 1) bwd_fixup<Cfg::BM, Cfg::BN>(P, tg, img_out, src.r0, src.c0, flag);
+    if (P.nslices > 1) bwd_fixup<Cfg::BM, Cfg::BN, Cfg::NT>(P, tg, img_out, src.r0, src.c0, flag);
 }
 
 template <class CX, class CY>
@@ -161,7 +161,7 @@ struct BwdLds {
 
 // Units [0, X.units) are dA units, the rest dB units.
 template <class CX, class CY>
-__global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads))) void
+__global__ __attribute__((amdgpu_flat_work_group_size(CX::NT, CX::NT))) void
 k_gemm_backward(BwdProblem X, BwdProblem Y)
 {
     __shared__ __attribute__((aligned(16))) char lds[BwdLds<CX, CY>::BYTES];
@@ -176,7 +176,7 @@ k_gemm_backward(BwdProblem X, BwdProblem Y)
 // One problem on its own: a strided batch of GEMMs C_i = op(A_i) op(B_i)
 // (a stride of 0 shares an operand; a C stride of 0 sums over the batch).
 template <class Cfg>
-__global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+__global__ __attribute__((amdgpu_flat_work_group_size(Cfg::NT, Cfg::NT),
                           amdgpu_waves_per_eu(Cfg::BM * Cfg::BN >= 128 * 128 ? 2 : 1, 8))) void
 k_gemm_strided(BwdProblem P)
 {
@@ -196,7 +196,7 @@ hipError_t launch_strided(const BwdProblem &P, hipStream_t s)
 {
     typedef TileCfg<PREC, TA, TB, BT, BT> Cfg;
     if (P.units <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gemm_strided<Cfg>, dim3(P.units), dim3(kThreads), 0, s, P);
+    hipLaunchKernelGGL(k_gemm_strided<Cfg>, dim3(P.units), dim3(Cfg::NT), 0, s, P);
     return hipGetLastError();
 }
 
@@ -212,7 +212,8 @@ hipError_t launch_backward(const BwdProblem &X, const BwdProblem &Y, hipStream_t
     typedef TileCfg<PREC, kBwdTA[TA][TB][1], kBwdTB[TA][TB][1], BT, BT> CY;
     const int grid = X.units + Y.units;
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_gemm_backward<CX, CY>), dim3(grid), dim3(kThreads), 0, s, X, Y);
+    static_assert(CX::NT == CY::NT, "one workgroup size for both problems");
+    hipLaunchKernelGGL((k_gemm_backward<CX, CY>), dim3(grid), dim3(CX::NT), 0, s, X, Y);
     return hipGetLastError();
 }
 

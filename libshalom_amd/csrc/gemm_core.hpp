@@ -1,8 +1,9 @@
 // gemm_core.hpp -- the tile GEMM at the heart of every LibShalom entry point,
 // written for gfx950 (CDNA4): 64-wide wavefronts, MFMA matrix cores, LDS.
 //
-// One workgroup (4 waves, 256 threads) computes one BM x BN tile of
-// C = op(A) * op(B) (row-major, beta = 0 overwrite) over a K range.
+// One workgroup (8 waves for 128 x 128 tiles, 4 otherwise) computes one
+// BM x BN tile of C = op(A) * op(B) (row-major, beta = 0 overwrite) over a K
+// range.
 //
 // * Fused pack.  LibShalom packs operand panels inside its micro-kernels
 //   (the first M block writes B into SB while computing, test_temp.c:9-537,
@@ -41,7 +42,6 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 enum Prec : int { P_FP32 = 0, P_BF16X3 = 1, P_BF16 = 2, P_BF16X6 = 3, P_FP64 = 4 };
 
-constexpr int kThreads = 256;
 constexpr int kWave = 64;
 
 // One GEMM.  Leading dimensions are in elements of the storage.
@@ -208,10 +208,16 @@ struct TileCfg {
     static constexpr int BUF_BYTES = A_BYTES + B_BYTES;
     static constexpr int NBUF = (2 * BUF_BYTES <= 80 * 1024) ? 2 : 1;
     static constexpr int LDS_BYTES = NBUF * BUF_BYTES;
-    static constexpr int WM_MAX = BM / 32, WN_MAX = BN / 32;  // sub-tiles per wave (2x2 grid fits)
+    // Workgroup size: eight waves for the 128 x 128 tiles, four otherwise.
+    // The waves form a (NW/2) x 2 grid over the tile; each owns WM x WN
+    // sub-tiles of 16 x 16.
+    static constexpr int NT = (BM == 128 && BN == 128) ? 512 : 256;
+    static constexpr int NW = NT / 64;
+    static constexpr int WM = BM / 16 / (NW / 2), WN = BN / 16 / 2;
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
-    static_assert(WM_MAX * WN_MAX <= 16, "at most 16 sub-tiles per wave");
+    static_assert(WM >= 1 && WN >= 1 && WM * (NW / 2) * 16 == BM, "the wave grid covers the tile");
+    static_assert(WM * WN <= 16, "at most 16 sub-tiles per wave");
     static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "panels stay 16-byte aligned");
 };
 
@@ -233,13 +239,13 @@ __device__ __forceinline__ int kpos(int k) { return (k & 3) * NSLOT + (k >> 2); 
 // LANEROW (BF16 modes, row-contiguous operand): lane-per-row 4-byte loads,
 // each thread holding VEC consecutive k of one row.
 // ---------------------------------------------------------------------------
-template <typename T, int NP, int ROWS, int BK, bool KCONTIG>
+template <typename T, int NP, int ROWS, int BK, bool KCONTIG, int NT>
 struct Stage {
     typedef PanelInfo<T, NP, ROWS, BK, KCONTIG> P;
     static constexpr int VEC = 16 / sizeof(T);
     static constexpr bool LANEROW = NP > 0 && !KCONTIG;
     static constexpr int NVEC = ROWS * BK / VEC;
-    static constexpr int PER = (NVEC + kThreads - 1) / kThreads;
+    static constexpr int PER = (NVEC + NT - 1) / NT;
     static constexpr int ES = sizeof(T);
     T v[PER][VEC];
 
@@ -274,10 +280,10 @@ struct Stage {
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(KCONTIG ? base + k0 : base + (size_t)k0 * ld, nrec);
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int idx = tid + i * kThreads;
+            const int idx = tid + i * NT;
             int row, k;
             coords(idx, row, k);
-            const bool act = (NVEC % kThreads == 0) | (idx < NVEC);
+            const bool act = (NVEC % NT == 0) | (idx < NVEC);
             if constexpr (KCONTIG) {
                 buf_load16<T>(rs, oob_unless(act & (k < kc), (row * ld + k) * ES), v[i]);
 #pragma unroll
@@ -301,8 +307,8 @@ struct Stage {
         constexpr int NSLOT = BK / 4;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
-            const int idx = tid + i * kThreads;
-            if (!((NVEC % kThreads == 0) || idx < NVEC)) continue;
+            const int idx = tid + i * NT;
+            if (!((NVEC % NT == 0) || idx < NVEC)) continue;
             int row, k;
             coords(idx, row, k);
             if constexpr (P::LAYOUT == L_ROWK) {
@@ -341,20 +347,20 @@ struct Tile {
     typedef typename Cfg::T T;
     typedef typename Cfg::Acc Acc;
     static constexpr int BM = Cfg::BM, BN = Cfg::BN, BK = Cfg::BK;
-    static constexpr int WM = Cfg::WM_MAX, WN = Cfg::WN_MAX;
-    typedef Stage<T, Cfg::NP, BM, BK, Cfg::A_KCONTIG> StageA;
-    typedef Stage<T, Cfg::NP, BN, BK, Cfg::B_KCONTIG> StageB;
+    static constexpr int WM = Cfg::WM, WN = Cfg::WN;
+    typedef Stage<T, Cfg::NP, BM, BK, Cfg::A_KCONTIG, Cfg::NT> StageA;
+    typedef Stage<T, Cfg::NP, BN, BK, Cfg::B_KCONTIG, Cfg::NT> StageB;
 
     Acc acc[WM][WN];
-    int wr0, wc0;  // this wave's quadrant (WM x WN sub-tiles of 16x16) in 16-row/col units
-    bool live;     // the quadrant holds part of the job's C tile
+    int wr0, wc0;  // this wave's block (WM x WN sub-tiles of 16x16) in 16-row/col units
+    bool live;     // the block holds part of the job's C tile
 
-    // The four waves own the four quadrants of the tile.  Every sub-tile of
-    // a live quadrant is multiplied and stored whether or not it reaches into
-    // the job: rows and columns past the job are zeros in LDS, and their
-    // stores are masked by the buffer range.  Issuing the MFMAs without
-    // per-sub-tile branches lets the LDS reads run ahead of them; a quadrant
-    // wholly outside the job skips its work with one uniform branch.
+    // The waves own the blocks of a (NW/2) x 2 grid over the tile.  Every
+    // sub-tile of a live block is multiplied and stored whether or not it
+    // reaches into the job: rows and columns past the job are zeros in LDS,
+    // and their stores are masked by the buffer range.  Issuing the MFMAs
+    // without per-sub-tile branches lets the LDS reads run ahead of them; a
+    // block wholly outside the job skips its work with one uniform branch.
     __device__ __forceinline__ void layout(int mt, int nt, int wave)
     {
         wr0 = (wave >> 1) * WM;

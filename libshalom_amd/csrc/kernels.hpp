@@ -16,7 +16,7 @@ constexpr int NPREC = 5;
 // for the float modes, fp64 for FP64); a reduction kernel sums the slices in
 // slice order, so results do not depend on scheduling.
 template <class Cfg>
-__global__ __launch_bounds__(kThreads) void k_gemm_tiles(GemmDesc d, int tiles_n, int ntiles, int kper,
+__global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n, int ntiles, int kper,
                                                         typename Cfg::T *partial)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(kThreads) void k_gemm_tiles(GemmDesc d, int tiles_n
 // 128 x 128 tiles (the register file then allows 256 per lane), which keeps
 // two job streams' loads in flight.
 template <class Cfg>
-__global__ __attribute__((amdgpu_flat_work_group_size(kThreads, kThreads),
+__global__ __attribute__((amdgpu_flat_work_group_size(Cfg::NT, Cfg::NT),
                           amdgpu_waves_per_eu(Cfg::BM * Cfg::BN >= 128 * 128 ? 2 : 1, 8))) void
 k_gemm_persistent(const GemmDesc *__restrict__ jobs, int njobs)
 {
@@ -81,7 +81,7 @@ hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, 
     const int tiles_m = (d.M + Cfg::BM - 1) / Cfg::BM, tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
     const int ntiles = tiles_m * tiles_n;
     dim3 grid(ntiles, ksplit);
-    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(kThreads), 0, s, d, tiles_n, ntiles, kper,
+    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(Cfg::NT), 0, s, d, tiles_n, ntiles, kper,
                        (typename Cfg::T *)partial);
     return hipGetLastError();
 }
@@ -89,7 +89,7 @@ hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, 
 template <class Cfg>
 hipError_t launch_persistent(const GemmDesc *jobs, int njobs, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gemm_persistent<Cfg>, dim3(grid), dim3(kThreads), 0, s, jobs, njobs);
+    hipLaunchKernelGGL(k_gemm_persistent<Cfg>, dim3(grid), dim3(Cfg::NT), 0, s, jobs, njobs);
     return hipGetLastError();
 }
 
@@ -97,7 +97,7 @@ template <class Cfg>
 int persistent_occupancy()
 {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_gemm_persistent<Cfg>, kThreads, 0) != hipSuccess) {
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_gemm_persistent<Cfg>, Cfg::NT, 0) != hipSuccess) {
         (void)hipGetLastError();
         return 0;
     }
