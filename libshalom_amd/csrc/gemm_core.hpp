@@ -142,20 +142,34 @@ __device__ __forceinline__ void buf_store1(__amdgpu_buffer_rsrc_t r, int off, T 
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, x), r, off, 0, 1);
 }
 
-// Split x into bf16 pieces, x ~= p0 + p1 + p2, each round-to-nearest-even of
-// the remaining residual (the residual subtraction is exact in FP32).
-template <int NP>
-__device__ __forceinline__ void split_bf16(float x, __bf16 (&p)[3])
+// Split four floats into NP bf16 pieces each, x ~= p0 + p1 + p2, every piece
+// the round-to-nearest-even of the remaining residual (the residual
+// subtraction is exact in FP32).  Pieces come packed two per dword (element
+// 2j in the low half), which is both the LDS layout and what
+// v_cvt_pk_bf16_f32 produces; a piece is widened back to FP32 by a shift or a
+// mask of its dword, so each float costs three VALU operations per piece
+// after the first.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b)
 {
-    __bf16 h = (__bf16)x;
-    p[0] = h;
-    if constexpr (NP >= 2) {
-        float r = x - (float)h;
-        __bf16 l = (__bf16)r;
-        p[1] = l;
-        if constexpr (NP >= 3) {
-            float r2 = r - (float)l;
-            p[2] = (__bf16)r2;
+    return __builtin_bit_cast(unsigned, __builtin_convertvector((f32x2){a, b}, bf16x2));
+}
+
+template <int NP>
+__device__ __forceinline__ void split4_bf16(const float (&x)[4], u32x2 (&p)[3])
+{
+    float r[4] = {x[0], x[1], x[2], x[3]};
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+        const unsigned w0 = pack_bf16(r[0], r[1]), w1 = pack_bf16(r[2], r[3]);
+        p[pl] = u32x2{w0, w1};
+        if (pl + 1 < NP) {
+            r[0] -= __builtin_bit_cast(float, w0 << 16);
+            r[1] -= __builtin_bit_cast(float, w0 & 0xffff0000u);
+            r[2] -= __builtin_bit_cast(float, w1 << 16);
+            r[3] -= __builtin_bit_cast(float, w1 & 0xffff0000u);
         }
     }
 }
@@ -217,6 +231,11 @@ struct TileCfg {
     static constexpr int NT = (BM == 128 && BN == 128) ? 512 : 256;
     static constexpr int NW = NT / 64;
     static constexpr int WM = BM / 16 / (NW / 2), WN = BN / 16 / 2;
+    // Waves per SIMD the persistent kernel is compiled for: four for the
+    // eight-wave tiles of FP32 / BF16 / BF16X3 (two workgroups per CU, at
+    // most 128 registers), two for the other 128 x 128 tiles.
+    static constexpr int PERSIST_WAVES = (NT == 512 && PREC != P_BF16X6 && PREC != P_FP64) ? 4
+                                         : (BM * BN >= 128 * 128 ? 2 : 1);
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
     static_assert(WM >= 1 && WN >= 1 && WM * (NW / 2) * 16 == BM, "the wave grid covers the tile");
@@ -328,15 +347,11 @@ struct Stage {
                 static_assert(sizeof(T) == 4, "split modes take FP32 inputs");
                 __bf16 *lds = (__bf16 *)dst;
                 constexpr int PLANE = ROWS * P::LDKH;
-                __bf16 p[VEC][3];
-#pragma unroll
-                for (int e = 0; e < VEC; ++e) split_bf16<NP>(v[i][e], p[e]);
+                u32x2 p[3];
+                split4_bf16<NP>(v[i], p);
                 // four consecutive k of one row: one 8-byte write per plane
 #pragma unroll
-                for (int pl = 0; pl < NP; ++pl) {
-                    bf16x4 w = {p[0][pl], p[1][pl], p[2][pl], p[3][pl]};
-                    *(bf16x4 *)(lds + pl * PLANE + row * P::LDKH + k) = w;
-                }
+                for (int pl = 0; pl < NP; ++pl) *(u32x2 *)(lds + pl * PLANE + row * P::LDKH + k) = p[pl];
             }
         }
     }
