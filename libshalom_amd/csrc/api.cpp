@@ -532,6 +532,27 @@ GemmDesc tile_job(int ta, int tb, const GemmDesc &g, int m0, int n0, int bm, int
 // kernels are bound by moving operands, so count elements moved.
 double job_cost(const GemmDesc &d) { return (double)(d.M + d.N) * std::max(d.K, 1) + (double)d.M * d.N; }
 
+// Order a bucket's jobs for a persistent launch of `grid` workgroups, where
+// workgroup b runs jobs b, b + grid, b + 2*grid, ...  A job's time is close to
+// its number of K chunks (each chunk is one step of the tile pipeline, the
+// same work whatever the tile's fill) plus about two thirds of a step for
+// writing its C tile.  Jobs are sorted by that estimate, longest first, and
+// dealt out in boustrophedon order (rounds of `grid` jobs, alternate rounds
+// reversed), so every workgroup's total is within about one job of the
+// others; plain round-robin over the sorted list gives workgroup 0 the
+// largest job of every round.
+void order_jobs(std::vector<GemmDesc> &jobs, int grid, int bk)
+{
+    auto est = [bk](const GemmDesc &d) { return (double)std::max(1, (d.K + bk - 1) / bk) + 0.65; };
+    std::stable_sort(jobs.begin(), jobs.end(), [&](const GemmDesc &x, const GemmDesc &y) {
+        const double ex = est(x), ey = est(y);
+        return ex != ey ? ex > ey : job_cost(x) > job_cost(y);
+    });
+    if (grid <= 1) return;
+    const size_t G = (size_t)grid;
+    for (size_t r0 = G; r0 + G <= jobs.size(); r0 += 2 * G) std::reverse(jobs.begin() + r0, jobs.begin() + r0 + G);
+}
+
 struct Bucket {
     int tile;
     std::vector<GemmDesc> jobs;
@@ -624,8 +645,9 @@ int time_assignment(int prec, int ta, int tb, const std::vector<GemmDesc> &descs
     size_t total = 0;
     std::vector<size_t> off(NTILE, 0);
     for (int t = 0; t < NTILE; ++t) {
-        std::stable_sort(b[t].begin(), b[t].end(),
-                         [](const GemmDesc &x, const GemmDesc &y) { return job_cost(x) > job_cost(y); });
+        PersistOccupancy occ = launch_table().persist_occ[prec][ta][tb][t];
+        const long slots = (long)std::max(1, occ ? occ() : 1) * num_cu;
+        order_jobs(b[t], (int)std::min<long>((long)b[t].size(), slots), esize == 8 ? 16 : 32);
         off[t] = total;
         total += ((b[t].size() * sizeof(GemmDesc)) + 255) & ~(size_t)255;
     }
@@ -833,20 +855,19 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
         for (int m0 = 0; m0 < d.M; m0 += ts.bm)
             for (int n0 = 0; n0 < d.N; n0 += ts.bn) b.jobs.push_back(tile_job(ta, tb, d, m0, n0, ts.bm, ts.bn, esize));
     }
-    // Per bucket: longest jobs first (workgroups take jobs round-robin, so
-    // each gets a similar mix), as many workgroups as can be resident.
+    // Per bucket: as many workgroups as can be resident, jobs dealt so that
+    // their totals balance.
     long total_jobs = 0;
     size_t total = 0;
     for (auto &b : buckets) {
         const long nj = (long)b.jobs.size();
         if (nj == 0) continue;
         if (nj > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
-        std::stable_sort(b.jobs.begin(), b.jobs.end(),
-                         [](const GemmDesc &x, const GemmDesc &y) { return job_cost(x) > job_cost(y); });
         total_jobs += nj;
         PersistOccupancy occ = launch_table().persist_occ[prec][ta][tb][b.tile];
         const long slots = (long)std::max(1, occ ? occ() : 1) * info.num_cu;
         b.grid = (int)std::min(nj, slots);
+        order_jobs(b.jobs, b.grid, f64 ? 16 : 32);
         b.off = total;
         total += ((size_t)nj * sizeof(GemmDesc) + 255) & ~(size_t)255;
     }

@@ -270,6 +270,7 @@ struct Stage {
     static constexpr int PER = (NVEC + NT - 1) / NT;
     static constexpr int ES = sizeof(T);
     T v[PER][VEC];
+    int kcut;  // valid k of the chunk (k-contiguous panels mask the rest at the LDS write)
 
     // First element (row, k) of the thread's group of VEC elements.
     __device__ __forceinline__ static void coords(int idx, int &row, int &k)
@@ -297,6 +298,7 @@ struct Stage {
                                          int tid)
     {
         const int kc = min(kend - k0, BK);  // valid k in this chunk
+        kcut = kc;
         const int nrec = (live && kc > 0 && R > 0) ? (KCONTIG ? ((R - 1) * ld + kc) * ES : ((kc - 1) * ld + R) * ES)
                                                    : 0;
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(KCONTIG ? base + k0 : base + (size_t)k0 * ld, nrec);
@@ -308,8 +310,6 @@ struct Stage {
             const bool act = (NVEC % NT == 0) | (idx < NVEC);
             if constexpr (KCONTIG) {
                 buf_load16<T>(rs, oob_unless(act & (k < kc), (row * ld + k) * ES), v[i]);
-#pragma unroll
-                for (int e = 1; e < VEC; ++e) v[i][e] = (k + e < kc) ? v[i][e] : T(0);
             } else if constexpr (LANEROW) {
                 const int hi = (act & (row < R)) ? 0 : kOOB;
                 const int off = (k * ld + row) * ES;
@@ -323,10 +323,24 @@ struct Stage {
         }
     }
 
-    // Registers -> LDS panel.
-    __device__ __forceinline__ void store(char *__restrict__ dst, int tid) const
+    // Registers -> LDS panel.  In a k-contiguous panel the elements of a
+    // group at k >= kcut (in range: the next columns of the row) are zeroed
+    // here, after the loads have landed, and only in a chunk that ends
+    // inside the tile's K (a uniform branch).
+    __device__ __forceinline__ void store(char *__restrict__ dst, int tid)
     {
         constexpr int NSLOT = BK / 4;
+        if constexpr (KCONTIG) {
+            if (kcut < BK) {
+#pragma unroll
+                for (int i = 0; i < PER; ++i) {
+                    int row, k;
+                    coords(tid + i * NT, row, k);
+#pragma unroll
+                    for (int e = 1; e < VEC; ++e) v[i][e] = (k + e < kcut) ? v[i][e] : T(0);
+                }
+            }
+        }
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int idx = tid + i * NT;
@@ -466,7 +480,7 @@ struct Tile {
     }
 
     template <class StageT>
-    __device__ __forceinline__ static void store_stage(const StageT &st, char *dst, int tid)
+    __device__ __forceinline__ static void store_stage(StageT &st, char *dst, int tid)
     {
         st.store(dst, tid);
     }
@@ -515,7 +529,7 @@ struct Tile {
     }
 
     // Registers -> LDS in the packed MFMA operand layout (transpose / split).
-    __device__ __forceinline__ static void chunk_to_lds(const StageA &sa, const StageB &sb, char *buf, int tid)
+    __device__ __forceinline__ static void chunk_to_lds(StageA &sa, StageB &sb, char *buf, int tid)
     {
         store_stage(sa, buf, tid);
         store_stage(sb, buf + Cfg::A_BYTES, tid);
