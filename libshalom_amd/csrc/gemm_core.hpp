@@ -174,6 +174,10 @@ __device__ __forceinline__ void split4_bf16(const float (&x)[4], u32x2 (&p)[3])
     }
 }
 
+// Position of k (0..31, bf16 elements) inside row `row` of a SPLIT panel
+// plane: the 16-byte chunk k / 8 is swizzled with bits 1-2 of the row.
+__device__ __forceinline__ int swz_k(int row, int k) { return (((k >> 3) ^ ((row >> 1) & 3)) << 3) | (k & 7); }
+
 // ---------------------------------------------------------------------------
 // LDS panel layouts of one operand chunk (ROWS x BK; "rows" are M for A, N
 // for B).  Which one is used depends on the mode and on whether the operand
@@ -184,10 +188,16 @@ __device__ __forceinline__ void split4_bf16(const float (&x)[4], u32x2 (&p)[3])
 //         so a lane's k steps are one contiguous 16-byte run.
 //   KROW  FP32/FP64, row-contiguous operand: [k][ROWS + 16], a straight copy
 //         of the 16-byte global loads; the MFMA reads one element per k step.
-//   SPLIT BF16 modes: NP planes of [row][BK + 8] bf16, natural k order; the
-//         threads hold four consecutive k of one row (16-byte loads for a
-//         k-contiguous operand, lane-per-row 4-byte loads otherwise) and write
-//         one 8-byte run per plane.
+//   SPLIT BF16 modes: NP planes of [row][BK] bf16 (64-byte rows, no
+//         padding), the four 16-byte chunks of row r (k = 8c..8c+7) stored
+//         in chunk order c ^ ((r >> 1) & 3); the threads hold four
+//         consecutive k of one row (16-byte loads for a k-contiguous operand,
+//         lane-per-row 4-byte loads otherwise) and write one 8-byte run per
+//         plane.  Measured on gfx950 (SQ_LDS_BANK_CONFLICT): with this
+//         swizzle the MFMA operand reads (lane (r, q): 16 bytes of row r,
+//         chunk q) and the 8-lanes-per-row writes are conflict-free, and the
+//         lane-per-row writes are 2-way; an 80-byte padded row made all three
+//         2-way.
 // ---------------------------------------------------------------------------
 enum Layout : int { L_ROWK = 0, L_KROW = 1, L_SPLIT = 2 };
 
@@ -197,7 +207,7 @@ struct PanelInfo {
     static constexpr int VEC = 16 / sizeof(T);
     static constexpr int LDK = BK + VEC;    // ROWK row length (elements)
     static constexpr int LDR = ROWS + 16;   // KROW row length (elements)
-    static constexpr int LDKH = BK + 8;     // SPLIT row length (bf16)
+    static constexpr int LDKH = BK;         // SPLIT row length (bf16)
     static constexpr int BYTES = LAYOUT == L_SPLIT ? NP * ROWS * LDKH * 2
                                  : LAYOUT == L_ROWK ? ROWS * LDK * (int)sizeof(T)
                                                     : BK * LDR * (int)sizeof(T);
@@ -219,11 +229,10 @@ struct TileCfg {
     static constexpr bool B_KCONTIG = TB;   // B^T[n][k] contiguous along k
     typedef PanelInfo<T, NP, BM, BK, A_KCONTIG> PA;
     typedef PanelInfo<T, NP, BN, BK, B_KCONTIG> PB;
-    static constexpr int LDKH = BK + 8;
     static constexpr int A_BYTES = PA::BYTES;
     static constexpr int B_BYTES = PB::BYTES;
     static constexpr int BUF_BYTES = A_BYTES + B_BYTES;
-    static constexpr int NBUF = (2 * BUF_BYTES <= 80 * 1024) ? 2 : 1;
+    static constexpr int NBUF = (2 * BUF_BYTES <= 96 * 1024) ? 2 : 1;
     static constexpr int LDS_BYTES = NBUF * BUF_BYTES;
     // Workgroup size: eight waves for the 128 x 128 tiles, four otherwise.
     // The waves form a (NW/2) x 2 grid over the tile; each owns WM x WN
@@ -365,7 +374,7 @@ struct Stage {
                 split4_bf16<NP>(v[i], p);
                 // four consecutive k of one row: one 8-byte write per plane
 #pragma unroll
-                for (int pl = 0; pl < NP; ++pl) *(u32x2 *)(lds + pl * PLANE + row * P::LDKH + k) = p[pl];
+                for (int pl = 0; pl < NP; ++pl) *(u32x2 *)(lds + pl * PLANE + row * P::LDKH + swz_k(row, k)) = p[pl];
             }
         }
     }
@@ -449,7 +458,8 @@ struct Tile {
                 }
             }
         } else {
-            constexpr int NP = Cfg::NP, LDKH = Cfg::LDKH;
+            constexpr int NP = Cfg::NP, LDKH = Cfg::PA::LDKH;
+            static_assert(Cfg::PB::LDKH == LDKH, "one split row length");
             const __bf16 *As = (const __bf16 *)buf;
             const __bf16 *Bs = (const __bf16 *)(buf + Cfg::A_BYTES);
             bf16x8 af[WM][NP];
@@ -457,7 +467,7 @@ struct Tile {
             for (int a = 0; a < WM; ++a)
 #pragma unroll
                 for (int pl = 0; pl < NP; ++pl)
-                    af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + ((wr0 + a) * 16 + r) * LDKH + q * 8);
+                    af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + ((wr0 + a) * 16 + r) * LDKH + swz_k(r, q * 8));
             // partial products (A piece, B piece), smallest first
             constexpr int NPROD = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
             constexpr int PA[6] = {NP == 3 ? 2 : 1, NP == 3 ? 1 : 0, NP == 3 ? 0 : 0, 1, 0, 0};
@@ -467,7 +477,7 @@ struct Tile {
                 bf16x8 bfr[NP];
 #pragma unroll
                 for (int pl = 0; pl < NP; ++pl)
-                    bfr[pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + ((wc0 + b) * 16 + r) * LDKH + q * 8);
+                    bfr[pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + ((wc0 + b) * 16 + r) * LDKH + swz_k(r, q * 8));
 #pragma unroll
                 for (int p = 0; p < NPROD; ++p) {
                     const int ia = NP == 1 ? 0 : PA[p], ib = NP == 1 ? 0 : PB[p];
