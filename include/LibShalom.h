@@ -97,6 +97,21 @@ int LibShalom_sgemm_mp_ex(int transa, int transb, float *C, const float *A, cons
 int LibShalom_dgemm_ex(int transa, int transb, double *C, const double *A, const double *B,
                        long M, long N, long K, LibShalom_stream_t stream);
 
+/* BLAS-style general GEMM (no reference counterpart; the reference's API has
+ * implicit leading dimensions and beta = 0, NN_LIB/LibShalom.h:22-42):
+ *   C = alpha * op(A) op(B) + beta * C, row-major,
+ * A is M x K (lda >= K) or, with transa, K x M (lda >= M); B is K x N
+ * (ldb >= N) or, with transb, N x K (ldb >= K); C is M x N (ldc >= N).  C is
+ * not read when beta is 0 (NaN or Inf in C do not propagate then).  Leading
+ * dimensions below the minimum return LIBSHALOM_INVALID_VALUE; at or above
+ * 2^21 elements, LIBSHALOM_NOT_SUPPORTED.  Asynchronous on `stream`. */
+int LibShalom_sgemm_general(int transa, int transb, long M, long N, long K, float alpha, const float *A,
+                            long lda, const float *B, long ldb, float beta, float *C, long ldc,
+                            int precision, LibShalom_stream_t stream);
+int LibShalom_dgemm_general(int transa, int transb, long M, long N, long K, double alpha, const double *A,
+                            long lda, const double *B, long ldb, double beta, double *C, long ldc,
+                            LibShalom_stream_t stream);
+
 /* Batched GEMMs of uneven sizes (no reference counterpart; the reference
  * makes one call per GEMM, e.g. benchmark/CP2K_kernels/LibShalom_dgemm.c:102-106).
  * A, B, C, M, N, K are HOST arrays of `count` entries (device pointers and

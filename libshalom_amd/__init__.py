@@ -19,7 +19,7 @@ import os
 __all__ = [
     "NoTrans", "Trans", "FP32", "BF16X3", "BF16", "BF16X6", "FP64",
     "LibShalomError", "lib", "sgemm", "sgemm_mp", "dgemm", "sgemm_ex", "sgemm_mp_ex",
-    "dgemm_ex", "set_thread_nums", "set_thread_ways", "get_thread_config",
+    "dgemm_ex", "sgemm_general", "dgemm_general", "set_thread_nums", "set_thread_ways", "get_thread_config",
     "set_precision", "get_precision", "BatchedPlan", "sgemm_batched", "dgemm_batched",
     "sgemm_backward", "sgemm_backward_strided", "sgemm_strided_batched", "autotune_batched",
     "autotune_reset", "query_tuned_tile", "query_plan", "query_batched_tile", "LIB_PATH",
@@ -73,6 +73,8 @@ SIGNATURES = {
     "LibShalom_sgemm_ex": (_I, [_I, _I, _P, _P, _P, _L, _L, _L, _I, _P]),
     "LibShalom_sgemm_mp_ex": (_I, [_I, _I, _P, _P, _P, _L, _L, _L, _I, _P]),
     "LibShalom_dgemm_ex": (_I, [_I, _I, _P, _P, _P, _L, _L, _L, _P]),
+    "LibShalom_sgemm_general": (_I, [_I, _I, _L, _L, _L, ct.c_float, _P, _L, _P, _L, ct.c_float, _P, _L, _I, _P]),
+    "LibShalom_dgemm_general": (_I, [_I, _I, _L, _L, _L, ct.c_double, _P, _L, _P, _L, ct.c_double, _P, _L, _P]),
     "LibShalom_sgemm_batched_plan": (_I, [ct.POINTER(_P), _I, _I, _P, _P, _P, _P, _P, _P, _L, _I]),
     "LibShalom_dgemm_batched_plan": (_I, [ct.POINTER(_P), _I, _I, _P, _P, _P, _P, _P, _P, _L]),
     "LibShalom_plan_execute": (_I, [_P, _P]),
@@ -208,6 +210,19 @@ def sgemm_mp_ex(transa, transb, C, A, B, M, N, K, precision=FP32, stream=None):
 def dgemm_ex(transa, transb, C, A, B, M, N, K, stream=None):
     _check(_lib.LibShalom_dgemm_ex(transa, transb, _ptr(C), _ptr(A), _ptr(B), M, N, K,
                                    _stream(stream)), "LibShalom_dgemm_ex")
+
+
+def sgemm_general(transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, precision=FP32, stream=None):
+    """C = alpha * op(A) op(B) + beta * C, row-major, explicit leading dimensions
+    (BLAS-style; no reference counterpart).  C is not read when beta is 0."""
+    _check(_lib.LibShalom_sgemm_general(transa, transb, M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb, beta,
+                                        _ptr(C), ldc, precision, _stream(stream)), "LibShalom_sgemm_general")
+
+
+def dgemm_general(transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, stream=None):
+    """FP64 counterpart of sgemm_general (FP64 MFMA)."""
+    _check(_lib.LibShalom_dgemm_general(transa, transb, M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb, beta,
+                                        _ptr(C), ldc, _stream(stream)), "LibShalom_dgemm_general")
 
 
 def sgemm_backward(transa, transb, dC, A, B, dA, dB, M, N, K, precision=FP32, stream=None):

@@ -200,9 +200,21 @@ GemmDesc make_desc(int ta, int tb, const void *A, const void *B, void *C, long M
     return d;
 }
 
-// One GEMM on `stream`: plan tiles / split-K and launch.
-int run_single(int prec, int ta, int tb, void *C, const void *A, const void *B, long M, long N, long K,
-               bool mp, hipStream_t stream)
+// Leading dimensions of a row-major GEMM: A is M x K (lda >= K) or, with
+// transa, K x M (lda >= M); B is K x N (ldb >= N) or, with transb, N x K
+// (ldb >= K); C is M x N (ldc >= N).  Every one is at least 1.
+int check_lds(int ta, int tb, long M, long N, long K, long lda, long ldb, long ldc)
+{
+    const long a_min = std::max(1L, ta ? M : K), b_min = std::max(1L, tb ? K : N), c_min = std::max(1L, N);
+    if (lda < a_min || ldb < b_min || ldc < c_min) return LIBSHALOM_INVALID_VALUE;
+    if (lda >= kMaxLd || ldb >= kMaxLd || ldc >= kMaxLd) return LIBSHALOM_NOT_SUPPORTED;
+    return LIBSHALOM_SUCCESS;
+}
+
+// One GEMM on `stream`, C = alpha * op(A) op(B) + beta * C with explicit
+// leading dimensions: plan tiles / split-K and launch.
+int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B, long M, long N, long K, long lda,
+                long ldb, long ldc, double alpha, double beta, bool mp, hipStream_t stream)
 {
     int dev;
     DeviceInfo info;
@@ -212,6 +224,9 @@ int run_single(int prec, int ta, int tb, void *C, const void *A, const void *B, 
     const bool f64 = prec == P_FP64;
     const int esize = f64 ? 8 : 4;
     GemmDesc d = make_desc(ta, tb, A, B, C, M, N, K, esize);
+    d.lda = (int)lda;
+    d.ldb = (int)ldb;
+    d.ldc = (int)ldc;
     SinglePlan p = plan_single(prec, M, N, K, mp, info.num_cu);
     TilesLaunch fn = launch_table().tiles[prec][ta][tb][p.tile];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
@@ -220,19 +235,27 @@ int run_single(int prec, int ta, int tb, void *C, const void *A, const void *B, 
         st = workspace(dev, stream, (size_t)p.ksplit * M * N * esize, &partial);
         if (st) return st;
     }
-    if (fn(d, p.ksplit, p.kper, partial, stream) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    if (fn(d, p.ksplit, p.kper, partial, alpha, beta, stream) != hipSuccess) return LIBSHALOM_HIP_ERROR;
     if (p.ksplit > 1) {
         const long mn = M * N;
         const int blocks = (int)std::min<long>((mn + 255) / 256, 4L * info.num_cu);
         if (f64)
-            hipLaunchKernelGGL(k_splitk_reduce<double>, dim3(blocks), dim3(256), 0, stream,
-                               (const double *)partial, (double *)C, mn, p.ksplit);
+            hipLaunchKernelGGL(k_splitk_reduce<double>, dim3(blocks), dim3(256), 0, stream, (const double *)partial,
+                               (double *)C, M, N, ldc, p.ksplit, alpha, beta);
         else
-            hipLaunchKernelGGL(k_splitk_reduce<float>, dim3(blocks), dim3(256), 0, stream,
-                               (const float *)partial, (float *)C, mn, p.ksplit);
+            hipLaunchKernelGGL(k_splitk_reduce<float>, dim3(blocks), dim3(256), 0, stream, (const float *)partial,
+                               (float *)C, M, N, ldc, p.ksplit, (float)alpha, (float)beta);
         if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
     }
     return LIBSHALOM_SUCCESS;
+}
+
+// One GEMM with the reference's semantics: implicit leading dimensions,
+// C overwritten (alpha 1, beta 0).
+int run_single(int prec, int ta, int tb, void *C, const void *A, const void *B, long M, long N, long K,
+               bool mp, hipStream_t stream)
+{
+    return run_general(prec, ta, tb, C, A, B, M, N, K, ta ? M : K, tb ? K : N, N, 1.0, 0.0, mp, stream);
 }
 
 int sync_null()
@@ -1021,6 +1044,32 @@ int LibShalom_dgemm_ex(int transa, int transb, double *C, const double *A, const
     int st = check_dims(transa, transb, M, N, K);
     if (!st) st = check_ptrs(C, A, B, M, N, K, false);
     if (!st) st = run_single(P_FP64, transa, transb, C, A, B, M, N, K, false, (hipStream_t)stream);
+    return set_status(st);
+}
+
+int LibShalom_sgemm_general(int transa, int transb, long M, long N, long K, float alpha, const float *A, long lda,
+                            const float *B, long ldb, float beta, float *C, long ldc, int precision,
+                            LibShalom_stream_t stream)
+{
+    int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
+    if (!st) st = check_dims(transa, transb, M, N, K);
+    if (!st) st = check_lds(transa, transb, M, N, K, lda, ldb, ldc);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    if (!st)
+        st = run_general(kprec(precision), transa, transb, C, A, B, M, N, K, lda, ldb, ldc, alpha, beta, false,
+                         (hipStream_t)stream);
+    return set_status(st);
+}
+
+int LibShalom_dgemm_general(int transa, int transb, long M, long N, long K, double alpha, const double *A, long lda,
+                            const double *B, long ldb, double beta, double *C, long ldc, LibShalom_stream_t stream)
+{
+    int st = check_dims(transa, transb, M, N, K);
+    if (!st) st = check_lds(transa, transb, M, N, K, lda, ldb, ldc);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    if (!st)
+        st = run_general(P_FP64, transa, transb, C, A, B, M, N, K, lda, ldb, ldc, alpha, beta, false,
+                         (hipStream_t)stream);
     return set_status(st);
 }
 

@@ -393,6 +393,11 @@ struct Tile {
     typedef Stage<T, Cfg::NP, BN, BK, Cfg::B_KCONTIG, Cfg::NT> StageB;
 
     Acc acc[WM][WN];
+    // Result scaling: with `scale` set, stores write alpha * product + beta *
+    // (the stored value), reading it only when beta is nonzero.  Only the
+    // single-GEMM tile kernel sets it; elsewhere it stays a constant false.
+    bool scale = false;
+    T alpha = T(1), beta = T(0);
     int wr0, wc0;  // this wave's block (WM x WN sub-tiles of 16x16) in 16-row/col units
     int wm, wn;    // sub-tiles of the block that reach into the job
     bool live;     // the block holds part of the job's C tile
@@ -687,8 +692,21 @@ struct Tile {
                 const int col = (wc0 + b) * 16 + r;
                 const int row = (wr0 + a) * 16 + acc_row(q, 0);
                 const int off = (row * ldo + col) * ES_OUT | (col < N ? 0 : kOOB);
+                if (scale) {
+                    OutT old[4] = {OutT(0), OutT(0), OutT(0), OutT(0)};
+                    if (beta != T(0)) {
 #pragma unroll
-                for (int i = 0; i < 4; ++i) buf_store1<OutT>(rs, off + i * rstep, (OutT)acc[a][b][i]);
+                        for (int i = 0; i < 4; ++i) old[i] = buf_load1<OutT>(rs, off + i * rstep);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        buf_store1<OutT>(rs, off + i * rstep,
+                                         beta != T(0) ? (OutT)(alpha * acc[a][b][i] + beta * old[i])
+                                                      : (OutT)(alpha * acc[a][b][i]));
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) buf_store1<OutT>(rs, off + i * rstep, (OutT)acc[a][b][i]);
+                }
             }
     }
 };

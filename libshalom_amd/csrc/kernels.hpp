@@ -14,10 +14,12 @@ constexpr int NPREC = 5;
 // One GEMM (or one K slice of it) over a 1-D grid of tiles.  With ksplit > 1,
 // blockIdx.y is the K slice and the tile goes to partial[slice] (M x N, fp32
 // for the float modes, fp64 for FP64); a reduction kernel sums the slices in
-// slice order, so results do not depend on scheduling.
+// slice order, so results do not depend on scheduling.  Without split-K the
+// tile is written as alpha * product + beta * C (C is not read when beta is 0).
 template <class Cfg>
 __global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n, int ntiles, int kper,
-                                                        typename Cfg::T *partial)
+                                                        typename Cfg::T *partial, typename Cfg::T alpha,
+                                                        typename Cfg::T beta)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
     const int b = xcd_remap(blockIdx.x, ntiles);
@@ -31,6 +33,9 @@ __global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n,
     else
         src.job = tile_job<Cfg>(d, m0, n0, kbeg, kend, partial + (size_t)blockIdx.y * d.M * d.N, d.N);
     Tile<Cfg> t;
+    t.scale = gridDim.y == 1 && !(alpha == typename Cfg::T(1) && beta == typename Cfg::T(0));
+    t.alpha = alpha;
+    t.beta = beta;
     t.stream(src, 0, 1, lds);
 }
 
@@ -48,22 +53,29 @@ k_gemm_persistent(const GemmDesc *__restrict__ jobs, int njobs)
     t.stream(src, blockIdx.x, gridDim.x, lds);
 }
 
-// C = sum over slices of partial[s], in slice order.
+// C = alpha * (sum over slices of partial[s], in slice order) + beta * C, for
+// the M x N partials and C with leading dimension ldc (C is not read when
+// beta is 0).
 template <typename T>
 __global__ __launch_bounds__(256) void k_splitk_reduce(const T *__restrict__ partial, T *__restrict__ C,
-                                                       long mn, int nslices)
+                                                       long M, long N, long ldc, int nslices, T alpha, T beta)
 {
+    const long mn = M * N;
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < mn; i += (long)gridDim.x * blockDim.x) {
         T s = partial[i];
         for (int j = 1; j < nslices; ++j) s += partial[(size_t)j * mn + i];
-        C[i] = s;
+        const long r = i / N, c = i - r * N;
+        T *o = C + r * ldc + c;
+        *o = beta == T(0) ? alpha * s : alpha * s + beta * *o;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Launcher table (host side).
 // ---------------------------------------------------------------------------
-typedef hipError_t (*TilesLaunch)(const GemmDesc &d, int ksplit, int kper, void *partial, hipStream_t s);
+// alpha / beta are read as the operand type (float or double).
+typedef hipError_t (*TilesLaunch)(const GemmDesc &d, int ksplit, int kper, void *partial, double alpha,
+                                  double beta, hipStream_t s);
 typedef hipError_t (*PersistLaunch)(const GemmDesc *jobs, int njobs, int grid, hipStream_t s);
 typedef int (*PersistOccupancy)();  // resident workgroups per CU, 0 on error
 
@@ -75,13 +87,15 @@ struct LaunchTable {
 LaunchTable &launch_table();
 
 template <class Cfg>
-hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, hipStream_t s)
+hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, double alpha, double beta,
+                        hipStream_t s)
 {
+    typedef typename Cfg::T T;
     const int tiles_m = (d.M + Cfg::BM - 1) / Cfg::BM, tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
     const int ntiles = tiles_m * tiles_n;
     dim3 grid(ntiles, ksplit);
-    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(Cfg::NT), 0, s, d, tiles_n, ntiles, kper,
-                       (typename Cfg::T *)partial);
+    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(Cfg::NT), 0, s, d, tiles_n, ntiles, kper, (T *)partial,
+                       (T)alpha, (T)beta);
     return hipGetLastError();
 }
 

@@ -87,6 +87,35 @@ def test_backward_argument_errors_before_device(kw, code):
         assert st == code
 
 
+@pytest.mark.parametrize("ta,tb,M,N,K,lda,ldb,ldc,code", [
+    (0, 0, 8, 6, 5, 4, 6, 6, 1),      # lda < K
+    (1, 0, 8, 6, 5, 7, 6, 6, 1),      # transposed A: lda < M
+    (0, 0, 8, 6, 5, 5, 5, 6, 1),      # ldb < N
+    (0, 1, 8, 6, 5, 5, 4, 6, 1),      # transposed B: ldb < K
+    (0, 0, 8, 6, 5, 5, 6, 5, 1),      # ldc < N
+    (0, 0, 8, 6, 0, 0, 6, 6, 1),      # K = 0 still needs lda >= 1
+    (0, 0, 8, 6, 5, 1 << 21, 6, 6, 5),  # offsets would overflow: not supported
+    (2, 0, 8, 6, 5, 5, 6, 6, 1),
+])
+def test_general_argument_errors_before_device(ta, tb, M, N, K, lda, ldb, ldc, code):
+    L = ls.lib()
+    st = L.LibShalom_sgemm_general(ta, tb, M, N, K, 1.0, 0x1000, lda, 0x1000, ldb, 0.0, 0x1000, ldc,
+                                   ls.FP32, None)
+    assert st == code and L.LibShalom_get_last_error() == code
+    st = L.LibShalom_dgemm_general(ta, tb, M, N, K, 1.0, 0x1000, lda, 0x1000, ldb, 0.0, 0x1000, ldc, None)
+    assert st == code
+    with pytest.raises(ls.LibShalomError) as e:
+        ls.sgemm_general(ta, tb, M, N, K, 2.0, 0x1000, lda, 0x1000, ldb, 1.0, 0x1000, ldc, ls.BF16X3, 0)
+    assert e.value.status == code
+
+
+def test_general_null_pointers_rejected():
+    L = ls.lib()
+    assert L.LibShalom_sgemm_general(0, 0, 4, 4, 4, 1.0, None, 4, 0x1000, 4, 0.0, 0x1000, 4, ls.FP32, None) == 2
+    assert L.LibShalom_dgemm_general(0, 0, 4, 4, 4, 1.0, 0x1000, 4, 0x1000, 4, 0.0, None, 4, None) == 2
+    assert L.LibShalom_sgemm_general(0, 0, 4, 4, 4, 1.0, 0x1000, 4, 0x1000, 4, 0.0, 0x1000, 4, 9, None) == 1
+
+
 def test_backward_empty_batch_is_a_no_op():
     L = ls.lib()
     assert L.LibShalom_sgemm_backward_strided(0, 0, None, None, None, None, None, 4, 4, 4, 0, 0, 0, 0,
