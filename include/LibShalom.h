@@ -38,7 +38,7 @@ typedef void *LibShalom_stream_t;
  * void and silently skip unsupported cases). */
 #define LIBSHALOM_SUCCESS 0
 #define LIBSHALOM_INVALID_VALUE 1   /* negative size, bad trans / precision flag */
-#define LIBSHALOM_INVALID_POINTER 2 /* NULL or non-device pointer for a non-empty operand */
+#define LIBSHALOM_INVALID_POINTER 2 /* NULL pointer for a non-empty operand */
 #define LIBSHALOM_HIP_ERROR 3       /* a HIP runtime call failed */
 #define LIBSHALOM_ALLOC_FAILED 4    /* workspace allocation failed */
 #define LIBSHALOM_NOT_SUPPORTED 5   /* size beyond the supported range (dims < 2^31) */
@@ -57,6 +57,10 @@ typedef void *LibShalom_stream_t;
 /* ------------------------------------------------------------------------
  * Reference API (synchronous: C is complete when the call returns).  Errors
  * leave C untouched and are reported by LibShalom_get_last_error().
+ * Operands may be device pointers or, as in the reference, host memory: host
+ * operands are staged through a device buffer (inputs copied in, C copied
+ * back), so the reference's harnesses run unchanged.  The other entry points
+ * take device-accessible pointers only.
  * ------------------------------------------------------------------------ */
 
 /* Replaces LibShalom_sgemm (NN_LIB/LibShalom.h:22, NN_SMM.c:5-27). */

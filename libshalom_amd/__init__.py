@@ -152,21 +152,36 @@ def _last(where: str):
 # ---------------------------------------------------------------------------
 # reference API (synchronous, null stream)
 # ---------------------------------------------------------------------------
+def _ref_ptr(x):
+    """Address of an operand of the reference-API calls, which also take host
+    memory as the reference does (the library stages it through the device):
+    a torch tensor on the GPU or the CPU, a C-contiguous numpy array, or an int."""
+    if hasattr(x, "__array_interface__") and not hasattr(x, "data_ptr"):
+        if not x.flags["C_CONTIGUOUS"]:
+            raise ValueError("operands must be contiguous row-major arrays")
+        return x.ctypes.data
+    if hasattr(x, "data_ptr") and getattr(x, "device", None) is not None and x.device.type == "cpu":
+        if not x.is_contiguous():
+            raise ValueError("operands must be contiguous row-major tensors")
+        return x.data_ptr()
+    return _ptr(x)
+
+
 def sgemm(transa, transb, C, A, B, M, N, K):
     """LibShalom_sgemm (NN_SMM.c:5): C = op(A) op(B), C overwritten."""
-    _lib.LibShalom_sgemm(transa, transb, _ptr(C), _ptr(A), _ptr(B), M, N, K)
+    _lib.LibShalom_sgemm(transa, transb, _ref_ptr(C), _ref_ptr(A), _ref_ptr(B), M, N, K)
     _last("LibShalom_sgemm")
 
 
 def sgemm_mp(transa, transb, C, A, B, M, N, K):
     """LibShalom_sgemm_mp (SMM_thread.c:75): irregular-shape path."""
-    _lib.LibShalom_sgemm_mp(transa, transb, _ptr(C), _ptr(A), _ptr(B), M, N, K)
+    _lib.LibShalom_sgemm_mp(transa, transb, _ref_ptr(C), _ref_ptr(A), _ref_ptr(B), M, N, K)
     _last("LibShalom_sgemm_mp")
 
 
 def dgemm(transa, transb, C, A, B, M, N, K):
     """LibShalom_dgemm (NN_SMM.c:30): FP64."""
-    _lib.LibShalom_dgemm(transa, transb, _ptr(C), _ptr(A), _ptr(B), M, N, K)
+    _lib.LibShalom_dgemm(transa, transb, _ref_ptr(C), _ref_ptr(A), _ref_ptr(B), M, N, K)
     _last("LibShalom_dgemm")
 
 

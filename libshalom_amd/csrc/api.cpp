@@ -495,13 +495,72 @@ int strided_run(int prec, int ta, int tb, float *C, const float *A, const float 
     return fn(P, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }
 
-int sgemm_sync(int ta, int tb, float *C, float *A, float *B, long M, long N, long K, bool mp)
+// ---------------------------------------------------------------------------
+// The reference-API calls (synchronous, null stream).  The reference takes
+// host pointers; here device pointers are used as they are, and operands in
+// host memory (pageable or pinned, which the GPU cannot address) are staged
+// through a per-device device buffer: inputs copied in, C copied back when
+// the product is done.  The staging buffer is held for the whole call.
+// ---------------------------------------------------------------------------
+struct Staging {
+    void *ptr = nullptr;
+    size_t size = 0;
+};
+std::mutex g_stage_mu;
+std::map<int, Staging> g_stage;
+
+template <typename T>
+int reference_call(int prec, int ta, int tb, T *C, T *A, T *B, long M, long N, long K, bool mp)
 {
     int st = check_dims(ta, tb, M, N, K);
-    if (!st) st = check_ptrs(C, A, B, M, N, K, true);
-    if (!st) st = run_single(kprec(g_precision.load()), ta, tb, C, A, B, M, N, K, mp, nullptr);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    if (st || M == 0 || N == 0) return st;
+    int dev;
+    DeviceInfo info;
+    st = current_device(&dev, &info);
+    if (st) return st;
+    const bool hC = !device_accessible(C);
+    const bool hA = K > 0 && !device_accessible(A);
+    const bool hB = K > 0 && !device_accessible(B);
+    if (!(hC || hA || hB)) {
+        st = run_single(prec, ta, tb, C, A, B, M, N, K, mp, nullptr);
+        return st ? st : sync_null();
+    }
+    auto round = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t bA = hA ? round((size_t)M * K * sizeof(T)) : 0;
+    const size_t bB = hB ? round((size_t)K * N * sizeof(T)) : 0;
+    const size_t bC = hC ? round((size_t)M * N * sizeof(T)) : 0;
+    std::lock_guard<std::mutex> lk(g_stage_mu);
+    Staging &sg = g_stage[dev];
+    if (sg.size < bA + bB + bC) {
+        if (sg.ptr) (void)hipFree(sg.ptr);  // hipFree waits for the device
+        sg.ptr = nullptr;
+        sg.size = 0;
+        if (hipMalloc(&sg.ptr, bA + bB + bC) != hipSuccess) {
+            (void)hipGetLastError();
+            sg.ptr = nullptr;
+            return LIBSHALOM_ALLOC_FAILED;
+        }
+        sg.size = bA + bB + bC;
+    }
+    char *base = (char *)sg.ptr;
+    T *dA = hA ? (T *)base : A;
+    T *dB = hB ? (T *)(base + bA) : B;
+    T *dC = hC ? (T *)(base + bA + bB) : C;
+    if (hA && hipMemcpy(dA, A, (size_t)M * K * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return (void)hipGetLastError(), LIBSHALOM_HIP_ERROR;
+    if (hB && hipMemcpy(dB, B, (size_t)K * N * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return (void)hipGetLastError(), LIBSHALOM_HIP_ERROR;
+    st = run_single(prec, ta, tb, dC, dA, dB, M, N, K, mp, nullptr);
     if (!st) st = sync_null();
-    return set_status(st);
+    if (!st && hC && hipMemcpy(C, dC, (size_t)M * N * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
+        return (void)hipGetLastError(), LIBSHALOM_HIP_ERROR;
+    return st;
+}
+
+int sgemm_sync(int ta, int tb, float *C, float *A, float *B, long M, long N, long K, bool mp)
+{
+    return set_status(reference_call<float>(kprec(g_precision.load()), ta, tb, C, A, B, M, N, K, mp));
 }
 
 }  // namespace
@@ -965,11 +1024,7 @@ void LibShalom_sgemm_mp(int transa, int transb, float *C, float *A, float *B, lo
 
 void LibShalom_dgemm(int transa, int transb, double *C, double *A, double *B, long M, long N, long K)
 {
-    int st = check_dims(transa, transb, M, N, K);
-    if (!st) st = check_ptrs(C, A, B, M, N, K, true);
-    if (!st) st = run_single(P_FP64, transa, transb, C, A, B, M, N, K, false, nullptr);
-    if (!st) st = sync_null();
-    set_status(st);
+    set_status(reference_call<double>(P_FP64, transa, transb, C, A, B, M, N, K, false));
 }
 
 void LibShalom_set_thread_nums(int num)

@@ -52,6 +52,40 @@ def test_hello_80_nt(dev):
     assert np.isfinite(C).all()
 
 
+def test_reference_api_host_pointers(dev):
+    """The reference's own ABI: Hello.c passes host buffers.  The reference-API
+    calls stage host operands through the device (all host, and mixed)."""
+    M = N = K = 80
+    A, B = oracle.hello_inputs(M, N, K)
+    C = np.full((M, N), np.nan, dtype=np.float32)
+    ls.sgemm(ls.NoTrans, ls.Trans, C, A, B, M, N, K)
+    bad, st = check_f32(C, A, B, 0, 1, M, N, K)
+    assert bad == 0, st
+    # A on the device, B and C on the host; then C on the device only
+    C2 = np.full((M, N), np.nan, dtype=np.float32)
+    ls.sgemm(ls.NoTrans, ls.Trans, C2, dev_tensor(A, dev), B, M, N, K)
+    assert np.array_equal(C2, C)
+    dC = nan_c(M, N, dev)
+    ls.sgemm(ls.NoTrans, ls.Trans, dC, A, B, M, N, K)
+    assert np.array_equal(dC.cpu().numpy(), C)
+    # sgemm_mp, tall-skinny, host operands
+    r = oracle.Drand48()
+    M2, N2, K2 = 4096, 16, 64
+    A2, B2 = r.matrix(M2, K2), r.matrix(K2, N2)
+    C3 = np.full((M2, N2), np.nan, dtype=np.float32)
+    ls.sgemm_mp(ls.NoTrans, ls.NoTrans, C3, A2, B2, M2, N2, K2)
+    bad, st = check_f32(C3, A2, B2, 0, 0, M2, N2, K2)
+    assert bad == 0, st
+    # dgemm with host float64 operands (a CP2K block shape)
+    M3, N3, K3 = 23, 26, 13
+    A3 = r.matrix(M3, K3).astype(np.float64)
+    B3 = r.matrix(K3, N3).astype(np.float64)
+    C4 = np.full((M3, N3), np.nan)
+    ls.dgemm(ls.NoTrans, ls.NoTrans, C4, A3, B3, M3, N3, K3)
+    Cref, absAB = oracle.dgemm_ref(0, 0, A3, B3, M3, N3, K3)
+    assert (np.abs(C4 - Cref.reshape(M3, N3)) <= 2 * K3 * U64 * absAB.reshape(M3, N3) + 1e-300).all()
+
+
 @pytest.mark.parametrize("ta,tb", TRANS)
 def test_small_sweep_8_to_128(dev, ta, tb):
     """benchmark/small_SGEMM: M = N = K = 8..128 step 8 (all four trans combos)."""
