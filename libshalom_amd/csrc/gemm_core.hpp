@@ -235,11 +235,15 @@ struct TileCfg {
     static constexpr int NBUF = (2 * BUF_BYTES <= 96 * 1024) ? 2 : 1;
     static constexpr int LDS_BYTES = NBUF * BUF_BYTES;
     // Workgroup size: eight waves for the 128 x 128 tiles, four otherwise.
-    // The waves form a (NW/2) x 2 grid over the tile; each owns WM x WN
-    // sub-tiles of 16 x 16.
+    // The waves form a GM x GN grid over the tile, each owning WM x WN
+    // sub-tiles of 16 x 16: two columns of waves for square tiles, one column
+    // (the rows split) for tall tiles and one row for wide ones, so the tall-
+    // and short-skinny GEMMs those tiles serve keep every wave on valid rows
+    // or columns.
     static constexpr int NT = (BM == 128 && BN == 128) ? 512 : 256;
     static constexpr int NW = NT / 64;
-    static constexpr int WM = BM / 16 / (NW / 2), WN = BN / 16 / 2;
+    static constexpr int GN = BM == BN ? 2 : (BM > BN ? 1 : NW), GM = NW / GN;
+    static constexpr int WM = BM / 16 / GM, WN = BN / 16 / GN;
     // Waves per SIMD the persistent kernel is compiled for: four for the
     // eight-wave tiles of FP32 / BF16 / BF16X3 (two workgroups per CU, at
     // most 128 registers), two for the other 128 x 128 tiles.
@@ -247,7 +251,7 @@ struct TileCfg {
                                          : (BM * BN >= 128 * 128 ? 2 : 1);
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
-    static_assert(WM >= 1 && WN >= 1 && WM * (NW / 2) * 16 == BM, "the wave grid covers the tile");
+    static_assert(WM >= 1 && WN >= 1 && WM * GM * 16 == BM && WN * GN * 16 == BN, "the wave grid covers the tile");
     static_assert(WM * WN <= 16, "at most 16 sub-tiles per wave");
     static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "panels stay 16-byte aligned");
 };
@@ -402,7 +406,7 @@ struct Tile {
     int wm, wn;    // sub-tiles of the block that reach into the job
     bool live;     // the block holds part of the job's C tile
 
-    // The waves own the blocks of a (NW/2) x 2 grid over the tile.  Every
+    // The waves own the blocks of the GM x GN wave grid.  Every
     // sub-tile of a live block is multiplied and stored whether or not it
     // reaches into the job: rows and columns past the job are zeros in LDS,
     // and their stores are masked by the buffer range.  Issuing the MFMAs
@@ -412,8 +416,8 @@ struct Tile {
     // the BF16 ones, skip the sub-tiles outside the job (uniform branches).
     __device__ __forceinline__ void layout(int mt, int nt, int wave)
     {
-        wr0 = (wave >> 1) * WM;
-        wc0 = (wave & 1) * WN;
+        wr0 = (wave / Cfg::GN) * WM;
+        wc0 = (wave % Cfg::GN) * WN;
         live = uniform(wr0 < mt && wc0 < nt) != 0;
         wm = min(WM, mt - wr0);
         wn = min(WN, nt - wc0);

@@ -88,6 +88,11 @@ inline SinglePlan plan_single(int prec, long M, long N, long K, bool /*mp*/, int
     const int kd = chunk_depth(prec);
     for (int t = 0; t < NTILE; ++t) {
         if (prec == P_FP64 && t == T128x128) continue;  // 128 accumulator registers: occupancy 1
+        // The 128 x 128 tile's eight waves own fixed 32 x 64 blocks: a GEMM
+        // with a dimension of at most 64 leaves half of them idle, which the
+        // FP32 MFMAs (the slowest per element) cannot afford; the tall and
+        // wide tiles split the rows (columns) over all their waves instead.
+        if (prec == P_FP32 && t == T128x128 && std::min(M, N) <= 64) continue;
         for (int ks = 1; ks <= 64; ks *= 2) {
             if (ks > 1 && K < (long)ks * 4 * kd) break;  // keep >= 4 chunks per slice
             const double c = plan_cost(prec, M, N, K, t, ks, num_cu);
