@@ -246,9 +246,11 @@ struct TileCfg {
     static constexpr int WM = BM / 16 / GM, WN = BN / 16 / GN;
     // Waves per SIMD the persistent kernel is compiled for: four for the
     // eight-wave tiles of FP32 / BF16 / BF16X3 (two workgroups per CU, at
-    // most 128 registers), two for the other 128 x 128 tiles.
-    static constexpr int PERSIST_WAVES = (NT == 512 && PREC != P_BF16X6 && PREC != P_FP64) ? 4
-                                         : (BM * BN >= 128 * 128 ? 2 : 1);
+    // most 128 registers), two for the other 128 x 128 tiles.  (The
+    // single-GEMM tile kernel, with its alpha / beta epilogue, would spill
+    // under the four-wave limit and keeps the default.)
+    static constexpr int WAVES_PER_EU = (NT == 512 && PREC != P_BF16X6 && PREC != P_FP64) ? 4
+                                        : (BM * BN >= 128 * 128 ? 2 : 1);
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
     static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
     static_assert(WM >= 1 && WN >= 1 && WM * GM * 16 == BM && WN * GN * 16 == BN, "the wave grid covers the tile");

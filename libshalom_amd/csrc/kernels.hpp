@@ -44,7 +44,7 @@ __global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n,
 // CU for the 128 x 128 tiles, which keeps two job streams' loads in flight.
 template <class Cfg>
 __global__ __attribute__((amdgpu_flat_work_group_size(Cfg::NT, Cfg::NT),
-                          amdgpu_waves_per_eu(Cfg::PERSIST_WAVES, 8))) void
+                          amdgpu_waves_per_eu(Cfg::WAVES_PER_EU, 8))) void
 k_gemm_persistent(const GemmDesc *__restrict__ jobs, int njobs)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];

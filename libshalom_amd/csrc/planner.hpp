@@ -20,19 +20,21 @@ struct SinglePlan {
     int kper;    // K extent of one slice (multiple of the chunk depth)
 };
 
-// Workgroups per CU for each (precision, tile), from the kernels' LDS and
-// register use on gfx950 (160 KB LDS, 512 VGPR+AGPR per SIMD lane).
+// Workgroups per CU of the single-GEMM tile kernel for each (precision,
+// tile), from its LDS and register use on gfx950 (160 KB LDS, 512 VGPR+AGPR
+// per SIMD lane; four waves per workgroup, eight for 128 x 128).
 inline int tile_occupancy(int prec, int tile)
 {
     //                                 32x32 64x64 128x128 128x32 128x64 32x128
-    static const int fp32[NTILE] = {8, 4, 2, 3, 2, 3};
-    static const int bf16[NTILE] = {8, 4, 1, 3, 2, 3};
-    static const int bf6[NTILE] = {5, 2, 1, 2, 2, 2};
-    static const int fp64[NTILE] = {8, 4, 1, 3, 2, 3};
+    static const int fp32[NTILE] = {7, 4, 1, 3, 2, 3};
+    static const int bf16[NTILE] = {8, 4, 2, 4, 2, 4};
+    static const int bf3[NTILE] = {8, 4, 1, 4, 2, 4};
+    static const int bf6[NTILE] = {6, 3, 1, 2, 2, 2};
+    static const int fp64[NTILE] = {7, 3, 1, 3, 2, 3};
     switch (prec) {
     case P_FP32: return fp32[tile];
-    case P_BF16:
-    case P_BF16X3: return bf16[tile];
+    case P_BF16: return bf16[tile];
+    case P_BF16X3: return bf3[tile];
     case P_BF16X6: return bf6[tile];
     default: return fp64[tile];
     }
