@@ -347,6 +347,34 @@ def bench_conv(args, dev):
                 "fwd_max_rel": float(f"{fe[0]:.3g}"), "dW_max_rel": float(f"{we[0]:.3g}"),
                 "dX_max_rel": float(f"{xe[0]:.3g}"),
             }
+            # implicit GEMM: the im2col matrix gathered by the operand loads from
+            # NCHW images (3 x 3, stride 1, padding 1), against torch's unfold
+            # followed by the same strided GEMM
+            Cin, side = Kc // 9, int(round(HW ** 0.5))
+            Ximg = torch.randn(batch, Cin, side, side, generator=g, device=dev)
+            ims = time_ms(lambda: ls.sconv2d_forward(Y, W, Ximg, batch, Cin, side, side, Cout, 3, 3, (1, 1),
+                                                     (1, 1), (1, 1), prec, stream),
+                          args.steps, args.warmup, stream)
+
+            def explicit():
+                cols = torch.nn.functional.unfold(Ximg, 3, padding=1).contiguous()
+                ls.sgemm_strided_batched(0, 0, Y, W, cols, Cout, HW, Kc, batch, 0, Kc * HW, Cout * HW, prec,
+                                         stream)
+            ems = time_ms(explicit, args.steps, args.warmup, stream)
+            ls.sconv2d_forward(Y, W, Ximg, batch, Cin, side, side, Cout, 3, 3, (1, 1), (1, 1), (1, 1), prec,
+                               stream)
+            torch.cuda.synchronize()
+            c64 = torch.nn.functional.unfold(Ximg[:1].double(), 3, padding=1)[0]
+            ie = rel_err(Y[0], W64 @ c64, W64.abs() @ c64.abs())
+            bi = 4.0 * (Cout * Kc + batch * Cin * HW + batch * Cout * HW)
+            row[name].update({
+                "implicit_fwd_ms": round(ims, 4), "implicit_fwd_tflops": round(ff / ims / 1e9, 2),
+                "implicit_fwd_roof_frac": round(ff / ims / 1e9 / roof_tflops(name, ff, bi), 3),
+                "explicit_unfold_plus_fwd_ms": round(ems, 4), "implicit_max_rel": float(f"{ie[0]:.3g}"),
+            })
+            log(f"conv {Cout}x{HW}x{Kc} {name}: implicit fwd {ims:.3f} ms ({ff / ims / 1e9:.1f} TF/s), "
+                f"unfold + fwd {ems:.3f} ms, max rel {ie[0]:.2g}")
+            del Ximg
             tot_f[name] = tot_f.get(name, 0.0) + ff + fb
             tot_ms[name] = tot_ms.get(name, 0.0) + fms + bms
             log(f"conv {Cout}x{HW}x{Kc} {name}: fwd {fms:.3f} ms ({ff / fms / 1e9:.1f} TF/s), "

@@ -175,6 +175,27 @@ int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, co
                                      long batch, long strideA, long strideB, long strideC,
                                      int precision, LibShalom_stream_t stream);
 
+/* 2-D convolution forward as an implicit GEMM (no reference counterpart: the
+ * reference's CNN benchmark hands LibShalom_sgemm_mp the GEMMs of im2col
+ * convolutions, M = Cout, N = H*W, K = Cin*9 --
+ * benchmark/CNN_kernels/LibShalom_sgemm_mp.c:86-116 -- with the im2col
+ * matrix already stored).  Y[b] (Cout x P x Q) = W (Cout x C*R*S) times the
+ * im2col matrix of X[b] (C*R*S x P*Q), which is never stored: the operand
+ * loads gather it from X.  X is NCHW (batch x C x H x W), W is OIHW
+ * (Cout x C x R x S), Y is batch x Cout x P x Q with
+ * P = (H + 2 pad_h - dil_h (R - 1) - 1) / stride_h + 1 (Q likewise).  The
+ * precision flags of the float GEMMs apply.  One launch; long reductions are
+ * cut into slices of whole channels and summed inside it in a fixed order.
+ * Asynchronous on `stream`.  LIBSHALOM_NOT_SUPPORTED when one image
+ * (C*H*W floats) reaches 2 GiB or P*Q*Q or (C*R*S + R*S)*R*S reaches 2^32. */
+int LibShalom_sconv2d_forward(float *Y, const float *W, const float *X, long batch, long C, long H, long Wd,
+                              long Cout, long R, long S, long stride_h, long stride_w, long pad_h, long pad_w,
+                              long dil_h, long dil_w, int precision, LibShalom_stream_t stream);
+/* P and Q of that convolution (LIBSHALOM_INVALID_VALUE for a bad geometry;
+ * sizes <= 0 mean the dilated filter does not fit the padded input). */
+int LibShalom_conv2d_output_size(long H, long Wd, long R, long S, long stride_h, long stride_w, long pad_h,
+                                 long pad_w, long dil_h, long dil_w, long *P, long *Q);
+
 /* Autotuner for batched plans (no reference counterpart).  It runs the given
  * batch of shapes on the device (synthetic operands in a scratch buffer; the
  * caller's memory is not touched), groups the GEMMs by max(M, N) and K

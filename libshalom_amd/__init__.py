@@ -85,6 +85,8 @@ SIGNATURES = {
     "LibShalom_sgemm_backward": (_I, [_I, _I, _P, _P, _P, _P, _P, _L, _L, _L, _I, _P]),
     "LibShalom_sgemm_strided_batched": (_I, [_I, _I, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _I, _P]),
     "LibShalom_sgemm_backward_strided": (_I, [_I, _I, _P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _I, _P]),
+    "LibShalom_sconv2d_forward": (_I, [_P, _P, _P] + [_L] * 13 + [_I, _P]),
+    "LibShalom_conv2d_output_size": (_I, [_L] * 10 + [_LP, _LP]),
     "LibShalom_autotune_batched": (_I, [_I, _I, _I, _P, _P, _P, _L, ct.POINTER(_I), _P]),
     "LibShalom_autotune_reset": (_I, []),
     "LibShalom_query_tuned_tile": (_I, [_I, _I, _I, _L, _L, _L, ct.POINTER(_I), ct.POINTER(_I)]),
@@ -263,6 +265,24 @@ def sgemm_backward_strided(transa, transb, dC, A, B, dA, dB, M, N, K, batch, str
                                                  _ptr(dB), M, N, K, batch, strideA, strideB, strideC,
                                                  precision, _stream(stream)),
            "LibShalom_sgemm_backward_strided")
+
+
+def conv2d_output_size(H, W, R, S, stride=(1, 1), padding=(0, 0), dilation=(1, 1)):
+    """(P, Q) of a convolution of an H x W input with an R x S filter."""
+    P, Q = ct.c_long(), ct.c_long()
+    _check(_lib.LibShalom_conv2d_output_size(H, W, R, S, stride[0], stride[1], padding[0], padding[1],
+                                             dilation[0], dilation[1], ct.byref(P), ct.byref(Q)),
+           "LibShalom_conv2d_output_size")
+    return P.value, Q.value
+
+
+def sconv2d_forward(Y, Wt, X, batch, C, H, W, Cout, R, S, stride=(1, 1), padding=(0, 0), dilation=(1, 1),
+                    precision=FP32, stream=None):
+    """Y[b] = conv2d(X[b], Wt) (NCHW input, OIHW filter) as an implicit GEMM, one launch; the
+    im2col matrix is gathered by the operand loads, never stored."""
+    _check(_lib.LibShalom_sconv2d_forward(_ptr(Y), _ptr(Wt), _ptr(X), batch, C, H, W, Cout, R, S, stride[0],
+                                          stride[1], padding[0], padding[1], dilation[0], dilation[1],
+                                          precision, _stream(stream)), "LibShalom_sconv2d_forward")
 
 
 def _arrays(Cs, As, Bs, Ms, Ns, Ks):

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <new>
 #include <string>
 #include <utility>
@@ -346,6 +347,7 @@ void bwd_side(BwdProblem &P, const void *L, long long sL, int lda, const void *R
     P.Cc = (int)Cc;
     P.red = (int)red;
     P.nimg = nimg;
+    P.kalign = 32;
 }
 
 // Cut the reductions of both problems into slices of about `len` (a
@@ -369,7 +371,7 @@ void bwd_slice(BwdProblem &P, int T, long len)
         P.slen = P.red;
     } else if (!shared || P.red >= len) {
         long spi = ceil_div(P.red, len);
-        const long slen = ceil_div(ceil_div(P.red, spi), 32) * 32;
+        const long slen = ceil_div(ceil_div(P.red, spi), P.kalign) * P.kalign;
         spi = ceil_div(P.red, slen);
         P.spi = (int)spi;
         P.slen = (int)slen;
@@ -493,6 +495,55 @@ int strided_run(int prec, int ta, int tb, float *C, const float *A, const float 
     StridedLaunch fn = bwd_table().strided[prec][ta][tb][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
     return fn(P, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
+}
+
+// Output size of a convolution along one axis: (in + 2 pad - dil (f - 1) - 1) / stride + 1
+// (floor division; <= 0 when the dilated filter does not fit).
+long conv_out_size(long in, long f, long stride, long pad, long dil)
+{
+    const long span = in + 2 * pad - dil * (f - 1) - 1;
+    return span < 0 ? 0 : span / stride + 1;
+}
+
+// ceil(2^32 / d) for the device's division by d (0 for d == 1).
+unsigned div_magic(long d) { return d == 1 ? 0u : (unsigned)((((unsigned long long)1 << 32) + d - 1) / d); }
+
+// Convolution forward as an implicit GEMM over a batch of NCHW images: the
+// same sliced tile launch as a strided batch whose B operand is each
+// image's im2col matrix, gathered by the operand loads.  Slices of the
+// reduction hold whole channels.
+int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch, long C, long H, long W, long Cout,
+                 long R, long S, long sh, long sw, long ph, long pw, long dh, long dw, long P, long Q, hipStream_t s)
+{
+    int dev;
+    DeviceInfo info;
+    int st = current_device(&dev, &info);
+    if (st) return st;
+    const long RS = R * S, K = C * RS, PQ = P * Q;
+    ConvGeom g;
+    g.H = (int)H;
+    g.W = (int)W;
+    g.S = (int)S;
+    g.RS = (int)RS;
+    g.Q = (int)Q;
+    g.sh = (int)sh;
+    g.sw = (int)sw;
+    g.ph = (int)ph;
+    g.pw = (int)pw;
+    g.dh = (int)dh;
+    g.dw = (int)dw;
+    g.mRS = div_magic(RS);
+    g.mS = div_magic(S);
+    g.mQ = div_magic(Q);
+    BwdProblem Pb;
+    bwd_side(Pb, Wt, 0, (int)K, X, C * H * W, 0, Y, Cout * PQ, (int)PQ, Cout, PQ, K, (int)batch);
+    Pb.kalign = (int)(32 / std::gcd(32L, RS) * RS);
+    BwdProblem *probs[1] = {&Pb};
+    const int T = plan_problems(probs, 1, std::min(Cout, PQ) >= 128, dev, info.num_cu, s);
+    if (T <= 0) return -T;
+    ConvLaunch fn = bwd_table().conv_fwd[prec][T == 128];
+    if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    return fn(Pb, g, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }
 
 // ---------------------------------------------------------------------------
@@ -1240,6 +1291,39 @@ int LibShalom_sgemm_strided_batched(int transa, int transb, float *C, const floa
     if ((M * N > 0 && !C) || (M * N > 0 && K > 0 && (!A || !B))) return set_status(LIBSHALOM_INVALID_POINTER);
     return set_status(strided_run(kprec(precision), transa, transb, C, A, B, M, N, K, (int)batch, strideA, strideB,
                                   strideC, (hipStream_t)stream));
+}
+
+int LibShalom_sconv2d_forward(float *Y, const float *Wt, const float *X, long batch, long C, long H, long W,
+                              long Cout, long R, long S, long stride_h, long stride_w, long pad_h, long pad_w,
+                              long dil_h, long dil_w, int precision, LibShalom_stream_t stream)
+{
+    if (!valid_precision(precision)) return set_status(LIBSHALOM_INVALID_VALUE);
+    if (batch < 0 || C <= 0 || H <= 0 || W <= 0 || Cout <= 0 || R <= 0 || S <= 0 || stride_h <= 0 ||
+        stride_w <= 0 || pad_h < 0 || pad_w < 0 || dil_h <= 0 || dil_w <= 0)
+        return set_status(LIBSHALOM_INVALID_VALUE);
+    const long P = conv_out_size(H, R, stride_h, pad_h, dil_h), Q = conv_out_size(W, S, stride_w, pad_w, dil_w);
+    if (P <= 0 || Q <= 0) return set_status(LIBSHALOM_INVALID_VALUE);
+    // 32-bit offsets within an image and exact multiply-high divisions
+    const double K = (double)C * R * S, img = (double)C * H * W, PQ = (double)P * Q;
+    if (batch > INT32_MAX || Cout > INT32_MAX || 4.0 * img >= 2147483648.0 || PQ >= 2147483648.0 ||
+        (K + R * S) * R * S >= 4294967296.0 || PQ * Q >= 4294967296.0 || 2.0 * (H + 2.0 * pad_h) >= 2147483648.0 ||
+        2.0 * (W + 2.0 * pad_w) >= 2147483648.0)
+        return set_status(LIBSHALOM_NOT_SUPPORTED);
+    if (batch == 0) return set_status(LIBSHALOM_SUCCESS);
+    if (!Y || !Wt || !X) return set_status(LIBSHALOM_INVALID_POINTER);
+    return set_status(conv_fwd_run(kprec(precision), Y, Wt, X, batch, C, H, W, Cout, R, S, stride_h, stride_w, pad_h,
+                                   pad_w, dil_h, dil_w, P, Q, (hipStream_t)stream));
+}
+
+int LibShalom_conv2d_output_size(long H, long W, long R, long S, long stride_h, long stride_w, long pad_h,
+                                 long pad_w, long dil_h, long dil_w, long *P, long *Q)
+{
+    if (H <= 0 || W <= 0 || R <= 0 || S <= 0 || stride_h <= 0 || stride_w <= 0 || pad_h < 0 || pad_w < 0 ||
+        dil_h <= 0 || dil_w <= 0)
+        return set_status(LIBSHALOM_INVALID_VALUE);
+    if (P) *P = conv_out_size(H, R, stride_h, pad_h, dil_h);
+    if (Q) *Q = conv_out_size(W, S, stride_w, pad_w, dil_w);
+    return set_status(LIBSHALOM_SUCCESS);
 }
 
 int LibShalom_autotune_batched(int precision, int transa, int transb, const long *M, const long *N, const long *K,

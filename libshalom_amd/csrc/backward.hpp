@@ -33,7 +33,9 @@ struct BwdProblem {
     int nslices;          // reduction slices per output tile
     int spi;              // > 0: slices per image (a slice lies inside one image)
     int gimg;             // spi == 0: whole images per slice
-    int slen;             // spi > 0: reduction length of a slice (multiple of the K chunk)
+    int slen;             // spi > 0: reduction length of a slice (multiple of kalign)
+    int kalign;           // slice lengths are multiples of this (the K chunk, and a
+                          // convolution's R*S so that a slice starts on a channel)
     int units;            // workgroups of this problem
     float *slots;         // nslices > 1: partial tiles, [tile][slice][BM * BN]
     int *counters;        // nslices > 1: arrivals per tile (zero between launches)
@@ -41,9 +43,13 @@ struct BwdProblem {
 
 // The segments of one unit's slice as a job stream (see Tile::stream).  The
 // problem's fields are copied in (scalar registers): a pointer to the kernel
-// argument would send it through private memory.
-template <class Cfg>
+// argument would send it through private memory.  With IM2COL_ the right
+// operand of every image is the im2col matrix of the convolution `geom`
+// over the image Rt + img * sR: the job's B is that image at the slice's
+// first channel and its ldb the tile's first output pixel.
+template <class Cfg, bool IM2COL_ = false>
 struct SliceSrc {
+    static constexpr bool IM2COL = IM2COL_;
     const void *L, *Rt;
     long long sL, sR;
     int lda, ldb;
@@ -51,8 +57,10 @@ struct SliceSrc {
     int img0, nseg;   // first image and number of segments
     int rho0, rho1;   // reduction range of every segment
     int r0, c0;       // output tile origin
+    long long boff;   // offset of the segment's first B element in its image
     void *out;        // where the tile goes: the output, or the unit's slot
     int ldo;
+    ConvGeom geom;
 
     __device__ __forceinline__ int count() const { return nseg; }
     __device__ __forceinline__ GemmDesc get(int j) const
@@ -63,13 +71,13 @@ struct SliceSrc {
         const T *Ri = (const T *)Rt + (long long)img * sR;
         GemmDesc d;
         d.A = Li + (Cfg::TA ? (long long)rho0 * lda + r0 : (long long)r0 * lda + rho0);
-        d.B = Ri + (Cfg::TB ? (long long)c0 * ldb + rho0 : (long long)rho0 * ldb + c0);
+        d.B = Ri + boff;
         d.C = out;
         d.M = min(Cfg::BM, R - r0);
         d.N = min(Cfg::BN, Cc - c0);
         d.K = rho1 - rho0;
         d.lda = lda;
-        d.ldb = ldb;
+        d.ldb = IM2COL ? c0 : ldb;
         d.ldc = ldo;
         d.flags = j + 1 < nseg ? kChain : 0;
         return d;
@@ -106,15 +114,16 @@ __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_o
     if (threadIdx.x == 0) P.counters[tg] = 0;  // ready for the next launch
 }
 
-template <class Cfg>
-__device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, int *flag)
+template <class Cfg, bool IM2COL = false>
+__device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, int *flag, const ConvGeom &geom = {})
 {
     const int tg = u / P.nslices, s = u - tg * P.nslices;
     const bool shared = P.sO == 0;
     const int img_out = shared ? 0 : tg / P.ntiles;
     const int t = shared ? tg : tg - img_out * P.ntiles;
     const int tr = t / P.tiles_c, tc = t - tr * P.tiles_c;
-    SliceSrc<Cfg> src;
+    SliceSrc<Cfg, IM2COL> src;
+    src.geom = geom;
     src.L = P.L;
     src.Rt = P.Rt;
     src.sL = P.sL;
@@ -142,6 +151,11 @@ __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, 
         src.rho0 = 0;
         src.rho1 = P.red;
     }
+    // a convolution slice starts on a channel (slices are multiples of R*S)
+    if constexpr (IM2COL)
+        src.boff = (long long)(src.rho0 / geom.RS) * geom.H * geom.W;
+    else
+        src.boff = Cfg::TB ? (long long)src.c0 * P.ldb + src.rho0 : (long long)src.rho0 * P.ldb + src.c0;
     if (P.nslices == 1) {
         src.out = (float *)P.O + (long long)img_out * P.sO + (long long)src.r0 * P.ldo + src.c0;
         src.ldo = P.ldo;
@@ -185,11 +199,34 @@ k_gemm_strided(BwdProblem P)
     bwd_unit<Cfg>(P, blockIdx.x, lds, &flag);
 }
 
+// Convolution forward as an implicit GEMM: image i's output (Cout x P*Q) is
+// the filter (Cout x C*R*S) times the im2col matrix of image i, whose
+// elements the B-operand loads gather from the image directly.
+template <class Cfg>
+__global__ __attribute__((amdgpu_flat_work_group_size(Cfg::NT, Cfg::NT),
+                          amdgpu_waves_per_eu(Cfg::BM * Cfg::BN >= 128 * 128 ? 2 : 1, 8))) void
+k_conv_fwd(BwdProblem P, ConvGeom g)
+{
+    __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
+    __shared__ int flag;
+    bwd_unit<Cfg, true>(P, blockIdx.x, lds, &flag, g);
+}
+
 // ---------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------
 typedef hipError_t (*BwdLaunch)(const BwdProblem &X, const BwdProblem &Y, hipStream_t s);
 typedef hipError_t (*StridedLaunch)(const BwdProblem &P, hipStream_t s);
+typedef hipError_t (*ConvLaunch)(const BwdProblem &P, const ConvGeom &g, hipStream_t s);
+
+template <int PREC, int BT>
+hipError_t launch_conv_fwd(const BwdProblem &P, const ConvGeom &g, hipStream_t s)
+{
+    typedef TileCfg<PREC, 0, 0, BT, BT> Cfg;
+    if (P.units <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_conv_fwd<Cfg>, dim3(P.units), dim3(Cfg::NT), 0, s, P, g);
+    return hipGetLastError();
+}
 
 template <int PREC, int TA, int TB, int BT>
 hipError_t launch_strided(const BwdProblem &P, hipStream_t s)
@@ -221,6 +258,7 @@ constexpr int NBWD_TILE = 2;  // 0: 64 x 64, 1: 128 x 128
 struct BwdTable {
     BwdLaunch fn[NPREC][2][2][NBWD_TILE];
     StridedLaunch strided[NPREC][2][2][NBWD_TILE];
+    ConvLaunch conv_fwd[NPREC][NBWD_TILE];
 };
 BwdTable &bwd_table();
 
@@ -243,6 +281,8 @@ void register_backward(BwdTable &t)
     t.strided[PREC][0][1][1] = &launch_strided<PREC, 0, 1, 128>;
     t.strided[PREC][1][0][1] = &launch_strided<PREC, 1, 0, 128>;
     t.strided[PREC][1][1][1] = &launch_strided<PREC, 1, 1, 128>;
+    t.conv_fwd[PREC][0] = &launch_conv_fwd<PREC, 64>;
+    t.conv_fwd[PREC][1] = &launch_conv_fwd<PREC, 128>;
 }
 
 void register_backward_fp32(BwdTable &t);

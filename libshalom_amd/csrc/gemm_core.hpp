@@ -54,6 +54,25 @@ struct GemmDesc {
     int flags;  // kChain: the next job of the workgroup's stream continues this accumulator
 };
 
+// A 2-D convolution whose im2col matrix is the B operand of a GEMM, generated
+// by the operand loads instead of stored.  Input X is NCHW, one image's
+// channels C x H x W; the filter is R x S; output pixel (p, q) of a P x Q
+// map reads X[c][p*sh - ph + r*dh][q*sw - pw + s*dw] for filter tap
+// (c, r, s), or 0 where that falls in the padding.  The im2col matrix of an
+// image is (C*R*S) x (P*Q): row t = (c*R + r)*S + s, column n = p*Q + q.
+struct ConvGeom {
+    int H, W;               // input height, width
+    int S, RS;              // filter width; filter height x width
+    int Q;                  // output width
+    int sh, sw, ph, pw, dh, dw;  // stride, padding, dilation
+    unsigned mRS, mS, mQ;   // ceil(2^32 / d) for d = RS, S, Q; 0 when d == 1
+};
+
+// x / d for the multiplier m = ceil(2^32 / d) (m == 0: d == 1).  Exact when
+// x * d < 2^32: with e = m*d - 2^32 < d, x*m / 2^32 = x/d + x*e / (d 2^32),
+// and x*e < 2^32 keeps the excess below 1/d.
+__device__ __forceinline__ int udiv(int x, unsigned m) { return m ? (int)__umulhi((unsigned)x, m) : x; }
+
 // GemmDesc::flags bit: the job is one segment of a longer reduction.  Its C
 // tile is not stored and the next job of the same stream (which must have the
 // same tile) adds to the same accumulators.
@@ -338,6 +357,42 @@ struct Stage {
         }
     }
 
+    // The chunk at k0 of a convolution's im2col panel (a row-contiguous
+    // operand: row = output pixel, k = filter tap).  X points at the job's
+    // first channel, n0 is the tile's first output pixel, N its pixels and
+    // kend the job's taps.  Every element is one 4-byte load whose offset
+    // is computed from its (pixel, tap); taps in the padding, rows past N
+    // and k past kend take an out-of-range offset and read 0.
+    __device__ __forceinline__ void load_im2col(const T *__restrict__ X, const ConvGeom &g, int n0, int N, int kend,
+                                                int k0, bool live, int tid)
+    {
+        static_assert(!KCONTIG, "the im2col operand is row-contiguous");
+        const int kc = min(kend - k0, BK);
+        kcut = kc;
+        const int HW = g.H * g.W;
+        const int nch = udiv(kend + g.RS - 1, g.mRS);  // channels the job spans
+        const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0 && N > 0) ? nch * HW * ES : 0);
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int idx = tid + i * NT;
+            int row, k;
+            coords(idx, row, k);
+            const bool act = (NVEC % NT == 0) | (idx < NVEC);
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const int rr = LANEROW ? row : row + e, kk = LANEROW ? k + e : k;
+                const int t = k0 + kk, n = n0 + rr;
+                const int c = udiv(t, g.mRS), rem = t - c * g.RS;
+                const int r = udiv(rem, g.mS), s = rem - r * g.S;
+                const int p = udiv(n, g.mQ), q = n - p * g.Q;
+                const int ih = p * g.sh - g.ph + r * g.dh, iw = q * g.sw - g.pw + s * g.dw;
+                const bool ok = act & (rr < N) & (kk < kc) & ((unsigned)ih < (unsigned)g.H) &
+                                ((unsigned)iw < (unsigned)g.W);
+                v[i][e] = buf_load1<T>(rx, ok ? (c * HW + ih * g.W + iw) * ES : kOOB);
+            }
+        }
+    }
+
     // Registers -> LDS panel.  In a k-contiguous panel the elements of a
     // group at k >= kcut (in range: the next columns of the row) are zeroed
     // here, after the loads have landed, and only in a chunk that ends
@@ -542,11 +597,18 @@ struct Tile {
 
     // Global -> registers: the K chunk at k0 of job d (live == false: a
     // placeholder load that returns zeros, keeping instruction counts static).
-    __device__ __forceinline__ static void load_chunk(StageA &sa, StageB &sb, const GemmDesc &d, int k0, bool live,
-                                                      int tid)
+    // A source with IM2COL set describes B as a convolution's im2col matrix:
+    // d.B is the image at the job's first channel and d.ldb the tile's first
+    // output pixel.
+    template <class Src>
+    __device__ __forceinline__ static void load_chunk(StageA &sa, StageB &sb, const Src &src, const GemmDesc &d,
+                                                      int k0, bool live, int tid)
     {
         sa.load((const T *)d.A, d.lda, d.M, d.K, k0, live, tid);
-        sb.load((const T *)d.B, d.ldb, d.N, d.K, k0, live, tid);
+        if constexpr (Src::IM2COL)
+            sb.load_im2col((const T *)d.B, src.geom, d.ldb, d.N, d.K, k0, live, tid);
+        else
+            sb.load((const T *)d.B, d.ldb, d.N, d.K, k0, live, tid);
     }
 
     // Registers -> LDS in the packed MFMA operand layout (transpose / split).
@@ -603,7 +665,7 @@ struct Tile {
                                          StageB &b1, char *lds, int bi, const Src &src, int stride, int tid,
                                          int lane, int wave)
     {
-        load_chunk(a0, b0, src.get(max(n2.job, 0)), n2.c * BK, n2.job >= 0, tid);
+        load_chunk(a0, b0, src, src.get(max(n2.job, 0)), n2.c * BK, n2.job >= 0, tid);
         // keep the loads ahead of the multiply in the schedule
         __builtin_amdgcn_sched_barrier(0);
         const GemmDesc dc = src.get(cur.job);
@@ -655,8 +717,8 @@ struct Tile {
         advance(n2, src, stride);
         StageA a0, a1;
         StageB b0, b1;
-        load_chunk(a0, b0, d0, 0, true, tid);
-        load_chunk(a1, b1, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
+        load_chunk(a0, b0, src, d0, 0, true, tid);
+        load_chunk(a1, b1, src, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
         begin(d0.M, d0.N, 0, 0, wave);
         chunk_to_lds(a0, b0, lds_buf(lds, 0), tid);
         __syncthreads();
@@ -719,6 +781,7 @@ struct Tile {
 
 // Job source over a device array of jobs.
 struct ArraySrc {
+    static constexpr bool IM2COL = false;
     const GemmDesc *__restrict__ jobs;
     int n;
     __device__ __forceinline__ int count() const { return n; }
@@ -738,6 +801,7 @@ struct ArraySrc {
 
 // Job source of exactly one job held in registers.
 struct OneSrc {
+    static constexpr bool IM2COL = false;
     GemmDesc job;
     __device__ __forceinline__ int count() const { return 1; }
     __device__ __forceinline__ GemmDesc get(int) const { return job; }

@@ -60,6 +60,7 @@ def lib():
         L.oracle_check_kv.argtypes = [ct.c_void_p, ct.c_int, _dp, _dp, _dp, _L, ct.c_double,
                                       ct.c_double, _dp]
         L.oracle_check_kv.restype = _L
+        L.oracle_sconv2d_ref.argtypes = [_fp, _fp, _dp, _dp] + [_L] * 15
         L.cpu_sgemm.argtypes = [ct.c_int, ct.c_int, _fp, _fp, _fp, _L, _L, _L]
         L.cpu_dgemm.argtypes = [ct.c_int, ct.c_int, _dp, _dp, _dp, _L, _L, _L]
         L.cpu_sgemm_batched.argtypes = [ct.c_int, ct.c_int, _L, ct.c_void_p, ct.c_void_p,
@@ -140,6 +141,29 @@ def dgemm_ref(transa, transb, A, B, M, N, K):
     S = np.empty((M, N), dtype=np.float64)
     lib().oracle_dgemm_ref(transa, transb, _p(A, _dp), _p(B, _dp), _p(C, _dp), _p(S, _dp), M, N, K)
     return C, S
+
+
+def conv_out(H, R, stride, pad, dil):
+    """Output size of a convolution along one axis (0 when the dilated filter does not fit)."""
+    span = H + 2 * pad - dil * (R - 1) - 1
+    return 0 if span < 0 else span // stride + 1
+
+
+def sconv2d_ref(X, Wt, stride=(1, 1), padding=(0, 0), dilation=(1, 1)):
+    """Long-double reference of the convolution of X (batch, C, H, W) with Wt (Cout, C, R, S).
+    Returns (Yref float64 [batch, Cout, P, Q], absWX float64, same shape)."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Wt = np.ascontiguousarray(Wt, dtype=np.float32)
+    batch, C, H, W = X.shape
+    Cout, C2, R, S = Wt.shape
+    assert C2 == C
+    P = conv_out(H, R, stride[0], padding[0], dilation[0])
+    Q = conv_out(W, S, stride[1], padding[1], dilation[1])
+    Y = np.empty((batch, Cout, P, Q), dtype=np.float64)
+    A = np.empty_like(Y)
+    lib().oracle_sconv2d_ref(_p(X, _fp), _p(Wt, _fp), _p(Y, _dp), _p(A, _dp), batch, C, H, W, Cout, R, S,
+                             stride[0], stride[1], padding[0], padding[1], dilation[0], dilation[1], P, Q)
+    return Y, A
 
 
 def check(C, Cref, absAB, K, u=U_F32):

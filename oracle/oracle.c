@@ -128,6 +128,43 @@ void oracle_dgemm_ref_flat(int transa, int transb, long count, const double *A, 
     }
 }
 
+/* 2-D convolution, NCHW input X (batch x C x H x W), OIHW filter Wt
+ * (Cout x C x R x S), output batch x Cout x P x Q:
+ *   Y[b][o][p][q] = sum_{c,r,s} Wt[o][c][r][s] * X[b][c][p*sh - ph + r*dh][q*sw - pw + s*dw]
+ * with taps in the padding contributing 0.  This is the product of the
+ * filter (Cout x C*R*S) and the im2col matrix of each image that the
+ * reference's CNN benchmark hands to LibShalom_sgemm_mp
+ * (benchmark/CNN_kernels/LibShalom_sgemm_mp.c:86-116: M = Cout, N = H*W,
+ * K = Cin*9 for 3 x 3, stride 1, padding 1).  Long double sums; absWX gets
+ * the sum of |Wt||X| for the elementwise bound, K = C*R*S. */
+void oracle_sconv2d_ref(const float *X, const float *Wt, double *Yref, double *absWX, long batch, long C, long H,
+                        long W, long Cout, long R, long S, long sh, long sw, long ph, long pw, long dh, long dw,
+                        long P, long Q)
+{
+    for (long b = 0; b < batch; ++b)
+        for (long o = 0; o < Cout; ++o)
+            for (long p = 0; p < P; ++p)
+                for (long q = 0; q < Q; ++q) {
+                    long double acc = 0.0L, aa = 0.0L;
+                    for (long c = 0; c < C; ++c)
+                        for (long r = 0; r < R; ++r) {
+                            const long ih = p * sh - ph + r * dh;
+                            if (ih < 0 || ih >= H) continue;
+                            for (long s = 0; s < S; ++s) {
+                                const long iw = q * sw - pw + s * dw;
+                                if (iw < 0 || iw >= W) continue;
+                                const long double w = Wt[((o * C + c) * R + r) * S + s];
+                                const long double x = X[((b * C + c) * H + ih) * W + iw];
+                                acc += w * x;
+                                aa += fabsl(w) * fabsl(x);
+                            }
+                        }
+                    const long i = ((b * Cout + o) * P + p) * Q + q;
+                    Yref[i] = (double)acc;
+                    absWX[i] = (double)aa;
+                }
+}
+
 static long check_core2(const void *Cv, int is_f64, const double *Cref, const double *absAB,
                         const double *Kelem, long n, long K, double u, double extra, double *stats)
 {

@@ -24,6 +24,13 @@
  *     Pinned by closed forms (all-ones KAT, evaluate/test.c:38-46),
  *     transpose identities and exact small-integer products in
  *     tests/test_oracle.py.
+ *   - oracle_sconv2d_ref: the convolution whose im2col GEMM the reference's
+ *     CNN benchmark times (benchmark/CNN_kernels/LibShalom_sgemm_mp.c:86-116,
+ *     M = Cout, N = H*W, K = Cin*9); the reference stores the im2col matrix,
+ *     the implicit-GEMM path gathers it.  Pinned against torch's float64
+ *     conv2d, against torch's unfold (im2col) followed by oracle_sgemm_ref,
+ *     and by closed forms (a one-tap filter shifts the input, all-ones
+ *     operands count the taps inside the image) in tests/test_oracle.py.
  *   - cpu_sgemm / cpu_dgemm (+ _batched): the reference's C row-major
  *     verification loop as a triple loop (i-k-j for NN, i-j-k dot for NT),
  *     OpenMP over rows or over batch entries.  Used only as the reported
@@ -66,6 +73,12 @@ void oracle_dgemm_ref_flat(int transa, int transb, long count, const double *A, 
                            const long *offA, const long *offB, const long *offC, const long *M,
                            const long *N, const long *K, double *Cref, double *absAB, double *Kelem);
 
+/* 2-D convolution (NCHW input, OIHW filter, stride / padding / dilation per
+ * axis), the GEMM of the filter and each image's im2col matrix; long double
+ * sums, absWX = sum |W||X| over the taps inside the image. */
+void oracle_sconv2d_ref(const float *X, const float *Wt, double *Yref, double *absWX, long batch, long C, long H,
+                        long W, long Cout, long R, long S, long sh, long sw, long ph, long pw, long dh, long dw,
+                        long P, long Q);
 /* Elementwise check against the reference.
  * stats[0] = max |C - Cref| / (2*K*u*absAB)   (<= 1 means within tolerance)
  * stats[1] = max |C - Cref| / absAB            (componentwise relative error)

@@ -273,3 +273,47 @@ def test_tune_file_is_loaded(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True,
                          cwd=os.path.dirname(os.path.dirname(ls.LIB_PATH)))
     assert out.stdout.split() == ["(128,", "128)", "(32,", "32)"]
+
+
+# (batch, C, H, W, Cout, R, S, sh, sw, ph, pw, dh, dw, precision) -> status
+CONV_ARG_CASES = [
+    ((-1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 1),   # negative batch
+    ((1, 0, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 1),    # no channels
+    ((1, 3, 8, 8, 4, 3, 3, 0, 1, 1, 1, 1, 1, 0), 1),    # stride 0
+    ((1, 3, 8, 8, 4, 3, 3, 1, 1, -1, 1, 1, 1, 0), 1),   # negative padding
+    ((1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 0, 1, 0), 1),    # dilation 0
+    ((1, 3, 2, 8, 4, 5, 3, 1, 1, 0, 1, 1, 1, 0), 1),    # filter taller than the input: P = 0
+    ((1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 7), 1),    # bad precision
+    ((1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 4), 1),    # FP64 has no convolution
+    ((1, 1 << 20, 32, 32, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 5),  # one image reaches 2 GiB
+    ((0, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 0),    # empty batch: nothing to do
+]
+
+
+@pytest.mark.parametrize("args,code", CONV_ARG_CASES)
+def test_conv_argument_errors_before_device(args, code):
+    L = ls.lib()
+    st = L.LibShalom_sconv2d_forward(0x1000, 0x1000, 0x1000, *args, None)
+    assert st == code and L.LibShalom_get_last_error() == code
+
+
+def test_conv_null_pointers_rejected():
+    L = ls.lib()
+    g = (1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, ls.FP32, None)
+    assert L.LibShalom_sconv2d_forward(None, 0x1000, 0x1000, *g) == 2
+    assert L.LibShalom_sconv2d_forward(0x1000, None, 0x1000, *g) == 2
+    assert L.LibShalom_sconv2d_forward(0x1000, 0x1000, None, *g) == 2
+
+
+@pytest.mark.parametrize("H,R,st,pad,dil", [(56, 3, 1, 1, 1), (7, 3, 1, 1, 1), (15, 3, 2, 1, 1), (15, 3, 1, 2, 2),
+                                            (5, 5, 1, 3, 1), (9, 4, 2, 1, 2), (3, 5, 1, 0, 1), (10, 1, 3, 0, 1)])
+def test_conv_output_size_matches_torch(H, R, st, pad, dil):
+    import torch
+    P, Q = ls.conv2d_output_size(H, H + 1, R, R, (st, st), (pad, pad), (dil, dil))
+    x = torch.zeros(1, 1, H, H + 1)
+    w = torch.zeros(1, 1, R, R)
+    if H + 2 * pad < dil * (R - 1) + 1:
+        assert P <= 0
+        return
+    y = torch.nn.functional.conv2d(x, w, stride=st, padding=pad, dilation=dil)
+    assert (P, Q) == tuple(y.shape[2:])

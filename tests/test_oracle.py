@@ -8,6 +8,11 @@
   transpose identities (NT(A,B) == NN(A,B^T)).
 * Checker: planted mistakes (transposed operand, one perturbed element, NaN,
   dropped k term) must be caught; the exactly rounded result must pass.
+* Reference convolution: against torch's float64 conv2d (stride, padding,
+  dilation, non-square filters), against torch's unfold (im2col) followed by
+  the reference GEMM (bit-identical), and by closed forms: a one-tap filter
+  copies or shifts the input, all-ones operands count the taps inside the
+  image.  A filter flipped along one axis must be caught.
 """
 import json
 import os
@@ -183,3 +188,95 @@ def test_workload_hello_inputs_match_oracle():
     A, B = hello_inputs(80, 80, 80)
     A0, B0 = oracle.hello_inputs(80, 80, 80)
     assert np.array_equal(A, A0) and np.array_equal(B, B0)
+
+
+# (batch, C, H, W, Cout, R, S, stride, padding, dilation)
+CONV_GEOMS = [
+    (2, 3, 9, 11, 5, 3, 3, (1, 1), (1, 1), (1, 1)),
+    (1, 4, 8, 8, 6, 1, 1, (1, 1), (0, 0), (1, 1)),
+    (2, 5, 13, 7, 6, 3, 2, (2, 1), (1, 0), (1, 1)),
+    (1, 3, 15, 15, 4, 3, 3, (1, 1), (2, 2), (2, 2)),
+    (3, 2, 6, 5, 3, 5, 5, (1, 1), (3, 3), (1, 1)),
+    (1, 2, 5, 9, 2, 2, 4, (3, 2), (0, 1), (1, 2)),
+]
+
+
+def _conv_inputs(g, seed):
+    batch, C, H, W, Cout, R, S = g[:7]
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-1, 1, (batch, C, H, W)).astype(np.float32)
+    Wt = rng.uniform(-1, 1, (Cout, C, R, S)).astype(np.float32)
+    return X, Wt
+
+
+@pytest.mark.parametrize("g", CONV_GEOMS)
+def test_conv_reference_matches_torch(g):
+    torch = pytest.importorskip("torch")
+    X, Wt = _conv_inputs(g, 7)
+    st, pad, dil = g[7:]
+    Y, A = oracle.sconv2d_ref(X, Wt, st, pad, dil)
+    Yt = torch.nn.functional.conv2d(torch.from_numpy(X).double(), torch.from_numpy(Wt).double(), stride=st,
+                                    padding=pad, dilation=dil).numpy()
+    At = torch.nn.functional.conv2d(torch.from_numpy(X).double().abs(), torch.from_numpy(Wt).double().abs(),
+                                    stride=st, padding=pad, dilation=dil).numpy()
+    assert Y.shape == Yt.shape
+    np.testing.assert_allclose(Y, Yt, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(A, At, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("g", CONV_GEOMS)
+def test_conv_reference_is_gemm_of_unfold(g):
+    torch = pytest.importorskip("torch")
+    X, Wt = _conv_inputs(g, 11)
+    batch, C, H, W, Cout, R, S = g[:7]
+    st, pad, dil = g[7:]
+    Y, _ = oracle.sconv2d_ref(X, Wt, st, pad, dil)
+    cols = torch.nn.functional.unfold(torch.from_numpy(X), (R, S), dilation=dil, padding=pad, stride=st).numpy()
+    K, PQ = C * R * S, Y.shape[2] * Y.shape[3]
+    assert cols.shape == (batch, K, PQ)
+    for b in range(batch):
+        Cb, _ = oracle.sgemm_ref(0, 0, Wt.reshape(Cout, K), np.ascontiguousarray(cols[b]), Cout, PQ, K)
+        assert np.array_equal(Cb, Y[b].reshape(Cout, PQ))
+
+
+def test_conv_reference_one_tap_filters():
+    X, _ = _conv_inputs((2, 3, 7, 9, 3, 3, 3), 3)
+    Wt = np.zeros((3, 3, 3, 3), dtype=np.float32)
+    for o in range(3):
+        Wt[o, o, 1, 1] = 1.0  # centre tap, padding 1: the identity
+    Y, _ = oracle.sconv2d_ref(X, Wt, (1, 1), (1, 1), (1, 1))
+    assert np.array_equal(Y, X.astype(np.float64))
+    Wt[:] = 0
+    for o in range(3):
+        Wt[o, (o + 1) % 3, 0, 0] = 1.0  # top-left tap: shift down-right by one, channel o+1
+    Y, _ = oracle.sconv2d_ref(X, Wt, (1, 1), (1, 1), (1, 1))
+    for o in range(3):
+        src = X[:, (o + 1) % 3].astype(np.float64)
+        assert np.array_equal(Y[:, o, 1:, 1:], src[:, :-1, :-1])
+        assert not Y[:, o, 0, :].any() and not Y[:, o, :, 0].any()
+    # stride 2, no padding: every other pixel
+    Wt[:] = 0
+    Wt[0, 0, 0, 0] = 1.0
+    Y, _ = oracle.sconv2d_ref(X, Wt[:1, :, :1, :1].copy(), (2, 2), (0, 0), (1, 1))
+    assert np.array_equal(Y[:, 0], X[:, 0, ::2, ::2].astype(np.float64))
+
+
+def test_conv_reference_all_ones_counts_taps():
+    C, H, W = 4, 6, 5
+    X = np.ones((1, C, H, W), dtype=np.float32)
+    Wt = np.ones((2, C, 3, 3), dtype=np.float32)
+    Y, A = oracle.sconv2d_ref(X, Wt, (1, 1), (1, 1), (1, 1))
+    rows = np.array([2] + [3] * (H - 2) + [2], dtype=np.float64)
+    cols = np.array([2] + [3] * (W - 2) + [2], dtype=np.float64)
+    expect = C * np.outer(rows, cols)
+    assert np.array_equal(Y[0, 0], expect) and np.array_equal(Y[0, 1], expect) and np.array_equal(A[0, 0], expect)
+
+
+def test_conv_reference_catches_flipped_filter():
+    g = CONV_GEOMS[0]
+    X, Wt = _conv_inputs(g, 5)
+    Y, A = oracle.sconv2d_ref(X, Wt, (1, 1), (1, 1), (1, 1))
+    Yf, _ = oracle.sconv2d_ref(X, np.ascontiguousarray(Wt[:, :, :, ::-1]), (1, 1), (1, 1), (1, 1))
+    K = g[1] * g[5] * g[6]
+    assert oracle.check(Y.astype(np.float32), Y, A, K)[0] == 0
+    assert oracle.check(Yf.astype(np.float32), Y, A, K)[0] > 0
