@@ -187,7 +187,8 @@ int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, co
  * precision flags of the float GEMMs apply.  One launch; long reductions are
  * cut into slices of whole channels and summed inside it in a fixed order.
  * Asynchronous on `stream`.  LIBSHALOM_NOT_SUPPORTED when one image
- * (C*H*W floats) reaches 2 GiB or P*Q*Q or (C*R*S + R*S)*R*S reaches 2^32. */
+ * (C*H*W floats) reaches 2 GiB, P*Q*Q or (C*R*S + R*S)*R*S reaches 2^32, or
+ * H + 2 pad_h (W + 2 pad_w) reaches 2^29. */
 int LibShalom_sconv2d_forward(float *Y, const float *W, const float *X, long batch, long C, long H, long Wd,
                               long Cout, long R, long S, long stride_h, long stride_w, long pad_h, long pad_w,
                               long dil_h, long dil_w, int precision, LibShalom_stream_t stream);

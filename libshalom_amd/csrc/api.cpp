@@ -1306,8 +1306,8 @@ int LibShalom_sconv2d_forward(float *Y, const float *Wt, const float *X, long ba
     // 32-bit offsets within an image and exact multiply-high divisions
     const double K = (double)C * R * S, img = (double)C * H * W, PQ = (double)P * Q;
     if (batch > INT32_MAX || Cout > INT32_MAX || 4.0 * img >= 2147483648.0 || PQ >= 2147483648.0 ||
-        (K + R * S) * R * S >= 4294967296.0 || PQ * Q >= 4294967296.0 || 2.0 * (H + 2.0 * pad_h) >= 2147483648.0 ||
-        2.0 * (W + 2.0 * pad_w) >= 2147483648.0)
+        (K + R * S) * R * S >= 4294967296.0 || PQ * Q >= 4294967296.0 || H + 2.0 * pad_h >= 536870912.0 ||
+        W + 2.0 * pad_w >= 536870912.0)
         return set_status(LIBSHALOM_NOT_SUPPORTED);
     if (batch == 0) return set_status(LIBSHALOM_SUCCESS);
     if (!Y || !Wt || !X) return set_status(LIBSHALOM_INVALID_POINTER);

@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace ls {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -71,7 +73,9 @@ struct ConvGeom {
 // x / d for the multiplier m = ceil(2^32 / d) (m == 0: d == 1).  Exact when
 // x * d < 2^32: with e = m*d - 2^32 < d, x*m / 2^32 = x/d + x*e / (d 2^32),
 // and x*e < 2^32 keeps the excess below 1/d.
-__device__ __forceinline__ int udiv(int x, unsigned m) { return m ? (int)__umulhi((unsigned)x, m) : x; }
+// Written as a select rather than a branch so that scalar uses stay
+// straight-line code.
+__device__ __forceinline__ int udiv(int x, unsigned m) { return (int)__umulhi((unsigned)x, m) + (m == 0u ? x : 0); }
 
 // GemmDesc::flags bit: the job is one segment of a longer reduction.  Its C
 // tile is not stored and the next job of the same stream (which must have the
@@ -292,14 +296,16 @@ __device__ __forceinline__ int kpos(int k) { return (k & 3) * NSLOT + (k >> 2); 
 // tile take an out-of-range offset and elements past the K / row tail are
 // zeroed by a select.  The buffer's num_records is the job's exact extent,
 // so a group running past the end of the matrix reads nothing beyond it.
-// LANEROW (BF16 modes, row-contiguous operand): lane-per-row 4-byte loads,
-// each thread holding VEC consecutive k of one row.
+// LANEROW (BF16 modes and the im2col operand; row-contiguous operands):
+// lane-per-row 4-byte loads, each thread holding VEC consecutive k of one
+// row, so the lanes of a wave share their k.
 // ---------------------------------------------------------------------------
-template <typename T, int NP, int ROWS, int BK, bool KCONTIG, int NT>
+template <typename T, int NP, int ROWS, int BK, bool KCONTIG, int NT, bool LR = (NP > 0 && !KCONTIG)>
 struct Stage {
     typedef PanelInfo<T, NP, ROWS, BK, KCONTIG> P;
     static constexpr int VEC = 16 / sizeof(T);
-    static constexpr bool LANEROW = NP > 0 && !KCONTIG;
+    static constexpr bool LANEROW = LR;
+    static_assert(!(LR && KCONTIG), "lane-per-row loads are for row-contiguous operands");
     static constexpr int NVEC = ROWS * BK / VEC;
     static constexpr int PER = (NVEC + NT - 1) / NT;
     static constexpr int ES = sizeof(T);
@@ -360,35 +366,61 @@ struct Stage {
     // The chunk at k0 of a convolution's im2col panel (a row-contiguous
     // operand: row = output pixel, k = filter tap).  X points at the job's
     // first channel, n0 is the tile's first output pixel, N its pixels and
-    // kend the job's taps.  Every element is one 4-byte load whose offset
-    // is computed from its (pixel, tap); taps in the padding, rows past N
-    // and k past kend take an out-of-range offset and read 0.
+    // kend the job's taps.  Every element is one 4-byte load; taps in the
+    // padding, rows past N and k past kend take an out-of-range offset and
+    // read 0.  The lanes of a wave hold one group of VEC taps (lane-per-row
+    // layout, ROWS a multiple of the wave).  Lane l decodes tap k0 + l % BK
+    // of the chunk (channel, filter row and column: one division each, all
+    // lanes at once) and the wave reads the decoded taps it holds from
+    // those lanes, so each element costs two adds and two range compares on
+    // top of its pixel's coordinates.  (Decoding in scalar registers made
+    // the CU's one scalar unit the bottleneck.)
     __device__ __forceinline__ void load_im2col(const T *__restrict__ X, const ConvGeom &g, int n0, int N, int kend,
                                                 int k0, bool live, int tid)
     {
-        static_assert(!KCONTIG, "the im2col operand is row-contiguous");
+        static_assert(LANEROW && ROWS % kWave == 0, "a wave's lanes share their taps");
         const int kc = min(kend - k0, BK);
         kcut = kc;
         const int HW = g.H * g.W;
         const int nch = udiv(kend + g.RS - 1, g.mRS);  // channels the job spans
         const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0 && N > 0) ? nch * HW * ES : 0);
+        static_assert(BK <= kWave, "one lane per tap of the chunk");
+        const int tl = k0 + (tid & (BK - 1));
+        const int tc = udiv(tl, g.mRS), trem = tl - tc * g.RS;
+        const int tr = udiv(trem, g.mS), ts = trem - tr * g.S;
+        // A tap past the job's K gets a row offset no pixel can take, so the
+        // row range check masks it: H + 2 ph < 2^29 (checked on the host)
+        // bounds every real ih0 and dy below 2^29 in magnitude, and any sum
+        // with one 2^29 sentinel stays in [2^29 - ph, 2^30] -- above H, no
+        // overflow.
+        const int ldy = tl < kend ? tr * g.dh : (1 << 29), ldx = ts * g.dw;
+        const int ltoff = (tc * HW + tr * g.dh * g.W + ldx) * ES;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int idx = tid + i * NT;
             int row, k;
             coords(idx, row, k);
             const bool act = (NVEC % NT == 0) | (idx < NVEC);
+            // this lane's output pixel (p, q) and the input position of tap (0, 0)
+            const int n = n0 + row;
+            const int p = udiv(n, g.mQ), q = n - p * g.Q;
+            // a lane whose row is past the tile (or past the panel) gets the
+            // same out-of-range row as a tap past K
+            const int ih0 = (act & (row < N)) ? p * g.sh - g.ph : (1 << 29), iw0 = q * g.sw - g.pw;
+            // byte offset of tap (0, 0) of channel 0 (modulo 2^32: only used
+            // where ih0 is a real row), kept opaque so that the compiler adds
+            // the tap's offset to it instead of re-deriving (ih0 + dy) * W
+            // with a quarter-rate multiply
+            int base = (int)(((unsigned)ih0 * (unsigned)g.W + (unsigned)iw0) * ES);
+            asm volatile("" : "+v"(base));
+            const int kw = uniform(k);
 #pragma unroll
             for (int e = 0; e < VEC; ++e) {
-                const int rr = LANEROW ? row : row + e, kk = LANEROW ? k + e : k;
-                const int t = k0 + kk, n = n0 + rr;
-                const int c = udiv(t, g.mRS), rem = t - c * g.RS;
-                const int r = udiv(rem, g.mS), s = rem - r * g.S;
-                const int p = udiv(n, g.mQ), q = n - p * g.Q;
-                const int ih = p * g.sh - g.ph + r * g.dh, iw = q * g.sw - g.pw + s * g.dw;
-                const bool ok = act & (rr < N) & (kk < kc) & ((unsigned)ih < (unsigned)g.H) &
-                                ((unsigned)iw < (unsigned)g.W);
-                v[i][e] = buf_load1<T>(rx, ok ? (c * HW + ih * g.W + iw) * ES : kOOB);
+                const int kk = kw + e;  // < BK: the lane that decoded this tap
+                const int dy = __builtin_amdgcn_readlane(ldy, kk), dx = __builtin_amdgcn_readlane(ldx, kk);
+                const int toff = __builtin_amdgcn_readlane(ltoff, kk);
+                const bool ok = ((unsigned)(ih0 + dy) < (unsigned)g.H) & ((unsigned)(iw0 + dx) < (unsigned)g.W);
+                v[i][e] = buf_load1<T>(rx, ok ? base + toff : kOOB);
             }
         }
     }
@@ -421,6 +453,12 @@ struct Stage {
                 T *lds = (T *)dst;
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) lds[row * P::LDK + kpos<NSLOT>(k + e)] = v[i][e];
+            } else if constexpr (P::LAYOUT == L_KROW && LANEROW) {
+                // VEC consecutive k of one row: one 4-byte write per k, the
+                // wave's lanes covering 64 consecutive rows of each k row
+                T *lds = (T *)dst + k * P::LDR + row;
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) lds[e * P::LDR] = v[i][e];
             } else if constexpr (P::LAYOUT == L_KROW) {
                 T *lds = (T *)dst + k * P::LDR + row;
                 if constexpr (sizeof(T) == 4)
@@ -452,6 +490,10 @@ struct Tile {
     static constexpr int WM = Cfg::WM, WN = Cfg::WN;
     typedef Stage<T, Cfg::NP, BM, BK, Cfg::A_KCONTIG, Cfg::NT> StageA;
     typedef Stage<T, Cfg::NP, BN, BK, Cfg::B_KCONTIG, Cfg::NT> StageB;
+    // B stage of an im2col source (lane-per-row loads in every mode)
+    typedef Stage<T, Cfg::NP, BN, BK, false, Cfg::NT, true> StageBI;
+    template <class Src>
+    using StageBFor = typename std::conditional<Src::IM2COL, StageBI, StageB>::type;
 
     Acc acc[WM][WN];
     // Result scaling: with `scale` set, stores write alpha * product + beta *
@@ -600,8 +642,8 @@ struct Tile {
     // A source with IM2COL set describes B as a convolution's im2col matrix:
     // d.B is the image at the job's first channel and d.ldb the tile's first
     // output pixel.
-    template <class Src>
-    __device__ __forceinline__ static void load_chunk(StageA &sa, StageB &sb, const Src &src, const GemmDesc &d,
+    template <class Src, class SB>
+    __device__ __forceinline__ static void load_chunk(StageA &sa, SB &sb, const Src &src, const GemmDesc &d,
                                                       int k0, bool live, int tid)
     {
         sa.load((const T *)d.A, d.lda, d.M, d.K, k0, live, tid);
@@ -612,7 +654,8 @@ struct Tile {
     }
 
     // Registers -> LDS in the packed MFMA operand layout (transpose / split).
-    __device__ __forceinline__ static void chunk_to_lds(StageA &sa, StageB &sb, char *buf, int tid)
+    template <class SB>
+    __device__ __forceinline__ static void chunk_to_lds(StageA &sa, SB &sb, char *buf, int tid)
     {
         store_stage(sa, buf, tid);
         store_stage(sb, buf + Cfg::A_BYTES, tid);
@@ -660,9 +703,9 @@ struct Tile {
     // One step of the chunk pipeline.  `cur` is in LDS buffer bi; its
     // successor n1 is in flight into stage (a1, b1); stage (a0, b0) is free and
     // receives n2.  Two chunks stay in flight while `cur` is multiplied.
-    template <class Src>
-    __device__ __forceinline__ bool step(Cursor &cur, Cursor &n1, Cursor &n2, StageA &a0, StageB &b0, StageA &a1,
-                                         StageB &b1, char *lds, int bi, const Src &src, int stride, int tid,
+    template <class Src, class SB>
+    __device__ __forceinline__ bool step(Cursor &cur, Cursor &n1, Cursor &n2, StageA &a0, SB &b0, StageA &a1,
+                                         SB &b1, char *lds, int bi, const Src &src, int stride, int tid,
                                          int lane, int wave)
     {
         load_chunk(a0, b0, src, src.get(max(n2.job, 0)), n2.c * BK, n2.job >= 0, tid);
@@ -716,7 +759,7 @@ struct Tile {
         Cursor n2 = n1;
         advance(n2, src, stride);
         StageA a0, a1;
-        StageB b0, b1;
+        StageBFor<Src> b0, b1;
         load_chunk(a0, b0, src, d0, 0, true, tid);
         load_chunk(a1, b1, src, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
         begin(d0.M, d0.N, 0, 0, wave);

@@ -286,6 +286,7 @@ CONV_ARG_CASES = [
     ((1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 7), 1),    # bad precision
     ((1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 4), 1),    # FP64 has no convolution
     ((1, 1 << 20, 32, 32, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 5),  # one image reaches 2 GiB
+    ((1, 1, 1 << 28, 1, 1, 1, 1, 1, 1, 1 << 27, 0, 1, 1, 0), 5),  # H + 2 pad reaches 2^29
     ((0, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 0),    # empty batch: nothing to do
 ]
 
