@@ -523,6 +523,7 @@ int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch
     ConvGeom g;
     g.H = (int)H;
     g.W = (int)W;
+    g.R = (int)R;
     g.S = (int)S;
     g.RS = (int)RS;
     g.Q = (int)Q;

@@ -64,7 +64,7 @@ struct GemmDesc {
 // image is (C*R*S) x (P*Q): row t = (c*R + r)*S + s, column n = p*Q + q.
 struct ConvGeom {
     int H, W;               // input height, width
-    int S, RS;              // filter width; filter height x width
+    int R, S, RS;           // filter height, width; R x S
     int Q;                  // output width
     int sh, sw, ph, pw, dh, dw;  // stride, padding, dilation
     unsigned mRS, mS, mQ;   // ceil(2^32 / d) for d = RS, S, Q; 0 when d == 1
@@ -363,66 +363,117 @@ struct Stage {
         }
     }
 
-    // The chunk at k0 of a convolution's im2col panel (a row-contiguous
-    // operand: row = output pixel, k = filter tap).  X points at the job's
-    // first channel, n0 is the tile's first output pixel, N its pixels and
-    // kend the job's taps.  Every element is one 4-byte load; taps in the
-    // padding, rows past N and k past kend take an out-of-range offset and
-    // read 0.  The lanes of a wave hold one group of VEC taps (lane-per-row
-    // layout, ROWS a multiple of the wave).  Lane l decodes tap k0 + l % BK
-    // of the chunk (channel, filter row and column: one division each, all
-    // lanes at once) and the wave reads the decoded taps it holds from
-    // those lanes, so each element costs two adds and two range compares on
-    // top of its pixel's coordinates.  (Decoding in scalar registers made
-    // the CU's one scalar unit the bottleneck.)
-    __device__ __forceinline__ void load_im2col(const T *__restrict__ X, const ConvGeom &g, int n0, int N, int kend,
-                                                int k0, bool live, int tid)
+    // Per-lane coordinates of the im2col rows (output pixels) this thread
+    // loads, fixed for one output tile: the input position (ih0, iw0) of
+    // tap (0, 0), its byte offset in channel 0, and -- for filters of at
+    // most 32 taps -- the mask of taps (bit r*S + s) that fall inside the
+    // image.  A row past the tile (or the panel) gets ih0 = 2^29, which no
+    // tap brings back into the image, and an empty mask.
+    struct Pixels {
+        int ih0[PER], iw0[PER], base[PER];
+        unsigned taps[PER];
+    };
+
+    __device__ __forceinline__ static Pixels pixels(const ConvGeom &g, int n0, int N, int tid)
     {
-        static_assert(LANEROW && ROWS % kWave == 0, "a wave's lanes share their taps");
-        const int kc = min(kend - k0, BK);
-        kcut = kc;
-        const int HW = g.H * g.W;
-        const int nch = udiv(kend + g.RS - 1, g.mRS);  // channels the job spans
-        const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0 && N > 0) ? nch * HW * ES : 0);
-        static_assert(BK <= kWave, "one lane per tap of the chunk");
-        const int tl = k0 + (tid & (BK - 1));
-        const int tc = udiv(tl, g.mRS), trem = tl - tc * g.RS;
-        const int tr = udiv(trem, g.mS), ts = trem - tr * g.S;
-        // A tap past the job's K gets a row offset no pixel can take, so the
-        // row range check masks it: H + 2 ph < 2^29 (checked on the host)
-        // bounds every real ih0 and dy below 2^29 in magnitude, and any sum
-        // with one 2^29 sentinel stays in [2^29 - ph, 2^30] -- above H, no
-        // overflow.
-        const int ldy = tl < kend ? tr * g.dh : (1 << 29), ldx = ts * g.dw;
-        const int ltoff = (tc * HW + tr * g.dh * g.W + ldx) * ES;
+        Pixels px;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
             const int idx = tid + i * NT;
             int row, k;
             coords(idx, row, k);
             const bool act = (NVEC % NT == 0) | (idx < NVEC);
-            // this lane's output pixel (p, q) and the input position of tap (0, 0)
             const int n = n0 + row;
             const int p = udiv(n, g.mQ), q = n - p * g.Q;
-            // a lane whose row is past the tile (or past the panel) gets the
-            // same out-of-range row as a tap past K
             const int ih0 = (act & (row < N)) ? p * g.sh - g.ph : (1 << 29), iw0 = q * g.sw - g.pw;
-            // byte offset of tap (0, 0) of channel 0 (modulo 2^32: only used
-            // where ih0 is a real row), kept opaque so that the compiler adds
-            // the tap's offset to it instead of re-deriving (ih0 + dy) * W
-            // with a quarter-rate multiply
+            // modulo 2^32: only used where ih0 is a real row; kept opaque so
+            // that the compiler adds a tap's offset to it instead of
+            // re-deriving (ih0 + dy) * W with a quarter-rate multiply
             int base = (int)(((unsigned)ih0 * (unsigned)g.W + (unsigned)iw0) * ES);
             asm volatile("" : "+v"(base));
-            const int kw = uniform(k);
+            unsigned taps = 0;
+            if (g.RS <= 32) {
+                unsigned cols = 0;
+                for (int s = 0; s < g.S; ++s) cols |= (unsigned)((unsigned)(iw0 + s * g.dw) < (unsigned)g.W) << s;
+                for (int r = 0; r < g.R; ++r)
+                    if ((unsigned)(ih0 + r * g.dh) < (unsigned)g.H) taps |= cols << (r * g.S);
+            }
+            px.ih0[i] = ih0;
+            px.iw0[i] = iw0;
+            px.base[i] = base;
+            px.taps[i] = taps;
+        }
+        return px;
+    }
+
+    // The chunk at k0 of a convolution's im2col panel (a row-contiguous
+    // operand: row = output pixel, k = filter tap) for the tile whose lanes
+    // are px.  X points at the job's first channel and kend is the job's
+    // taps.  Every element is one 4-byte load; taps in the padding, rows
+    // past the tile and taps past kend take an out-of-range offset and read
+    // 0.  The lanes of a wave hold one group of VEC taps (lane-per-row
+    // layout, ROWS a multiple of the wave).  Lane l decodes tap k0 + l % BK
+    // of the chunk (channel, filter row and column: one division each, all
+    // lanes at once) and the wave reads the decoded taps it holds from those
+    // lanes (v_readlane), so per element a filter of at most 32 taps costs a
+    // test of the tap's bit in the pixel's mask, a larger one two adds and
+    // two range compares.  (Decoding in scalar registers made the CU's one
+    // scalar unit the bottleneck.)  The offsets are formed first and the
+    // loads issued after, outside the branch on the filter size.
+    __device__ __forceinline__ void load_im2col(const T *__restrict__ X, const ConvGeom &g, const Pixels &px,
+                                                int kend, int k0, bool live, int tid)
+    {
+        static_assert(LANEROW && ROWS % kWave == 0, "a wave's lanes share their taps");
+        static_assert(BK <= kWave, "one lane per tap of the chunk");
+        const int kc = min(kend - k0, BK);
+        kcut = kc;
+        const int HW = g.H * g.W;
+        const int nch = udiv(kend + g.RS - 1, g.mRS);  // channels the job spans
+        const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0) ? nch * HW * ES : 0);
+        const int tl = k0 + (tid & (BK - 1));
+        const int tc = udiv(tl, g.mRS), trem = tl - tc * g.RS;
+        const int tr = udiv(trem, g.mS), ts = trem - tr * g.S;
+        const int ltoff = (tc * HW + tr * g.dh * g.W + ts * g.dw) * ES;
+        int off[PER][VEC];
+        if (g.RS <= 32) {
+            const unsigned lbit = tl < kend ? 1u << trem : 0u;  // no bit for a tap past K
 #pragma unroll
-            for (int e = 0; e < VEC; ++e) {
-                const int kk = kw + e;  // < BK: the lane that decoded this tap
-                const int dy = __builtin_amdgcn_readlane(ldy, kk), dx = __builtin_amdgcn_readlane(ldx, kk);
-                const int toff = __builtin_amdgcn_readlane(ltoff, kk);
-                const bool ok = ((unsigned)(ih0 + dy) < (unsigned)g.H) & ((unsigned)(iw0 + dx) < (unsigned)g.W);
-                v[i][e] = buf_load1<T>(rx, ok ? base + toff : kOOB);
+            for (int i = 0; i < PER; ++i) {
+                int row, k;
+                coords(tid + i * NT, row, k);
+                const int kw = uniform(k);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const unsigned bit = __builtin_amdgcn_readlane(lbit, kw + e);
+                    const int toff = __builtin_amdgcn_readlane(ltoff, kw + e);
+                    off[i][e] = (px.taps[i] & bit) ? px.base[i] + toff : kOOB;
+                }
+            }
+        } else {
+            // a tap past K gets the out-of-range row offset 2^29: H + 2 ph
+            // < 2^29 (checked on the host) keeps every real ih0 and dy below
+            // 2^29 in magnitude, so a sum with a sentinel lies in
+            // [2^29 - ph, 2^30] -- above H, without overflow
+            const int ldy = tl < kend ? tr * g.dh : (1 << 29), ldx = ts * g.dw;
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                int row, k;
+                coords(tid + i * NT, row, k);
+                const int kw = uniform(k);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const int dy = __builtin_amdgcn_readlane(ldy, kw + e), dx = __builtin_amdgcn_readlane(ldx, kw + e);
+                    const int toff = __builtin_amdgcn_readlane(ltoff, kw + e);
+                    const bool ok = ((unsigned)(px.ih0[i] + dy) < (unsigned)g.H) &
+                                    ((unsigned)(px.iw0[i] + dx) < (unsigned)g.W);
+                    off[i][e] = ok ? px.base[i] + toff : kOOB;
+                }
             }
         }
+#pragma unroll
+        for (int i = 0; i < PER; ++i)
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) v[i][e] = buf_load1<T>(rx, off[i][e]);
     }
 
     // Registers -> LDS panel.  In a k-contiguous panel the elements of a
@@ -641,14 +692,16 @@ struct Tile {
     // placeholder load that returns zeros, keeping instruction counts static).
     // A source with IM2COL set describes B as a convolution's im2col matrix:
     // d.B is the image at the job's first channel and d.ldb the tile's first
-    // output pixel.
+    // output pixel; every job of such a stream has the same output tile,
+    // whose per-lane pixel coordinates px are computed once.
     template <class Src, class SB>
-    __device__ __forceinline__ static void load_chunk(StageA &sa, SB &sb, const Src &src, const GemmDesc &d,
-                                                      int k0, bool live, int tid)
+    __device__ __forceinline__ static void load_chunk(StageA &sa, SB &sb, const Src &src,
+                                                      const typename SB::Pixels &px, const GemmDesc &d, int k0,
+                                                      bool live, int tid)
     {
         sa.load((const T *)d.A, d.lda, d.M, d.K, k0, live, tid);
         if constexpr (Src::IM2COL)
-            sb.load_im2col((const T *)d.B, src.geom, d.ldb, d.N, d.K, k0, live, tid);
+            sb.load_im2col((const T *)d.B, src.geom, px, d.K, k0, live, tid);
         else
             sb.load((const T *)d.B, d.ldb, d.N, d.K, k0, live, tid);
     }
@@ -705,10 +758,11 @@ struct Tile {
     // receives n2.  Two chunks stay in flight while `cur` is multiplied.
     template <class Src, class SB>
     __device__ __forceinline__ bool step(Cursor &cur, Cursor &n1, Cursor &n2, StageA &a0, SB &b0, StageA &a1,
-                                         SB &b1, char *lds, int bi, const Src &src, int stride, int tid,
+                                         SB &b1, const typename SB::Pixels &px, char *lds, int bi, const Src &src,
+                                         int stride, int tid,
                                          int lane, int wave)
     {
-        load_chunk(a0, b0, src, src.get(max(n2.job, 0)), n2.c * BK, n2.job >= 0, tid);
+        load_chunk(a0, b0, src, px, src.get(max(n2.job, 0)), n2.c * BK, n2.job >= 0, tid);
         // keep the loads ahead of the multiply in the schedule
         __builtin_amdgcn_sched_barrier(0);
         const GemmDesc dc = src.get(cur.job);
@@ -760,14 +814,16 @@ struct Tile {
         advance(n2, src, stride);
         StageA a0, a1;
         StageBFor<Src> b0, b1;
-        load_chunk(a0, b0, src, d0, 0, true, tid);
-        load_chunk(a1, b1, src, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
+        typename StageBFor<Src>::Pixels px;
+        if constexpr (Src::IM2COL) px = StageBFor<Src>::pixels(src.geom, d0.ldb, d0.N, tid);
+        load_chunk(a0, b0, src, px, d0, 0, true, tid);
+        load_chunk(a1, b1, src, px, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
         begin(d0.M, d0.N, 0, 0, wave);
         chunk_to_lds(a0, b0, lds_buf(lds, 0), tid);
         __syncthreads();
         for (;;) {
-            if (!step(cur, n1, n2, a0, b0, a1, b1, lds, 0, src, stride, tid, lane, wave)) break;
-            if (!step(cur, n1, n2, a1, b1, a0, b0, lds, 1, src, stride, tid, lane, wave)) break;
+            if (!step(cur, n1, n2, a0, b0, a1, b1, px, lds, 0, src, stride, tid, lane, wave)) break;
+            if (!step(cur, n1, n2, a1, b1, a0, b0, px, lds, 1, src, stride, tid, lane, wave)) break;
         }
     }
 
