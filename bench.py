@@ -47,6 +47,8 @@ def main():
                     help="batched plans with the fixed tile rule instead of the autotuned tiles")
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "conv", "mp", "sweep", "hello", "dgemm_batched"])
+    ap.add_argument("--conv-layers", default="",
+                    help="conv workload: comma-separated indices of the layers to run (default all four)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="issue one device-to-device copy of this many MiB first (untimed), a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -312,7 +314,8 @@ def bench_conv(args, dev):
     stream = torch.cuda.current_stream()
     batch = 32
     layers, tot_f, tot_ms = [], {}, {}
-    for (Cout, HW, Kc) in CONV_SHAPES:
+    picked = [int(x) for x in args.conv_layers.split(",")] if args.conv_layers else range(len(CONV_SHAPES))
+    for (Cout, HW, Kc) in [CONV_SHAPES[i] for i in picked]:
         g = torch.Generator(device=dev).manual_seed(Cout)
         W = torch.randn(Cout, Kc, generator=g, device=dev) * (2.0 / Kc) ** 0.5
         X = torch.randn(batch, Kc, HW, generator=g, device=dev)

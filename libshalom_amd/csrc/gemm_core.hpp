@@ -929,16 +929,4 @@ __device__ __forceinline__ GemmDesc tile_job(const GemmDesc &g, int m0, int n0, 
     return j;
 }
 
-// XCD-aware block remap: the dispatcher places workgroup b on XCD b % 8, so
-// give each XCD a contiguous run of logical blocks (neighbouring tiles share
-// operand panels through that XCD's L2).
-__device__ __forceinline__ int xcd_remap(int b, int nblocks)
-{
-    constexpr int X = 8;
-    const int per = nblocks / X;
-    const int xcd = b % X, local = b / X;
-    if (local >= per) return b;  // the last nblocks % 8 blocks keep their place
-    return xcd * per + local;
-}
-
 }  // namespace ls

@@ -17,12 +17,18 @@ constexpr int NPREC = 5;
 // slice order, so results do not depend on scheduling.  Without split-K the
 // tile is written as alpha * product + beta * C (C is not read when beta is 0).
 template <class Cfg>
-__global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n, int ntiles, int kper,
+__global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n, int kper,
                                                         typename Cfg::T *partial, typename Cfg::T alpha,
                                                         typename Cfg::T beta)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
-    const int b = xcd_remap(blockIdx.x, ntiles);
+    // Tiles in row-major order of blockIdx.x.  The dispatcher deals
+    // workgroups to the 8 XCDs round-robin, so the resident tiles of one XCD
+    // are every 8th of a band of rows: a compact set whose A and B panels
+    // share that XCD's L2.  (Remapping each XCD to a contiguous run of
+    // tiles made it sweep whole tile rows, every B panel through one L2,
+    // and was up to 5% slower on large grids.)
+    const int b = blockIdx.x;
     const int tm = b / tiles_n, tn = b - tm * tiles_n;
     const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
     const int kbeg = blockIdx.y * kper;
@@ -94,7 +100,7 @@ hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, 
     const int tiles_m = (d.M + Cfg::BM - 1) / Cfg::BM, tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
     const int ntiles = tiles_m * tiles_n;
     dim3 grid(ntiles, ksplit);
-    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(Cfg::NT), 0, s, d, tiles_n, ntiles, kper, (T *)partial,
+    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(Cfg::NT), 0, s, d, tiles_n, kper, (T *)partial,
                        (T)alpha, (T)beta);
     return hipGetLastError();
 }
