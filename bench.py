@@ -377,7 +377,43 @@ def bench_conv(args, dev):
             })
             log(f"conv {Cout}x{HW}x{Kc} {name}: implicit fwd {ims:.3f} ms ({ff / ims / 1e9:.1f} TF/s), "
                 f"unfold + fwd {ems:.3f} ms, max rel {ie[0]:.2g}")
-            del Ximg
+            # implicit backward (dX images and dW in one launch) against unfold + the
+            # fused stored-im2col backward + fold (col2im) of the column gradient
+            dXimg = torch.empty_like(Ximg)
+
+            def implicit_bwd():
+                ls.sconv2d_backward(dXimg, dW, dY, W, Ximg, batch, Cin, side, side, Cout, 3, 3, (1, 1), (1, 1),
+                                    (1, 1), prec, stream)
+            ibms = time_ms(implicit_bwd, args.steps, args.warmup, stream)
+
+            def explicit_bwd():
+                cols = torch.nn.functional.unfold(Ximg, 3, padding=1).contiguous()
+                ls.sgemm_backward_strided(0, 0, dY, W, cols, dW, dX, Cout, HW, Kc, batch, 0, Kc * HW, Cout * HW,
+                                          prec, stream)
+                torch.nn.functional.fold(dX, (side, side), 3, padding=1)
+            ebms = time_ms(explicit_bwd, args.steps, args.warmup, stream)
+            implicit_bwd()
+            torch.cuda.synchronize()
+            dc64 = W64.T @ dY64[0]
+            rx = torch.nn.functional.fold(dc64[None], (side, side), 3, padding=1)[0]
+            ax = torch.nn.functional.fold((W64.abs().T @ dY64[0].abs())[None], (side, side), 3, padding=1)[0]
+            dxe = rel_err(dXimg[0], rx, ax)
+            gW64 = torch.zeros_like(W64)
+            aW64 = torch.zeros_like(W64)
+            for i0 in range(0, batch, 8):
+                c64 = torch.nn.functional.unfold(Ximg[i0:i0 + 8].double(), 3, padding=1)
+                ct = c64.transpose(1, 2).contiguous()
+                gW64 += torch.matmul(dY64[i0:i0 + 8], ct).sum(0)
+                aW64 += torch.matmul(dY64[i0:i0 + 8].abs(), ct.abs()).sum(0)
+            dwe = rel_err(dW, gW64, aW64)
+            row[name].update({
+                "implicit_bwd_ms": round(ibms, 4), "implicit_bwd_tflops": round(fb / ibms / 1e9, 2),
+                "explicit_unfold_bwd_fold_ms": round(ebms, 4),
+                "implicit_dX_max_rel": float(f"{dxe[0]:.3g}"), "implicit_dW_max_rel": float(f"{dwe[0]:.3g}"),
+            })
+            log(f"conv {Cout}x{HW}x{Kc} {name}: implicit bwd {ibms:.3f} ms ({fb / ibms / 1e9:.1f} TF/s), "
+                f"unfold + bwd + fold {ebms:.3f} ms, max rel dX {dxe[0]:.2g} dW {dwe[0]:.2g}")
+            del Ximg, dXimg
             tot_f[name] = tot_f.get(name, 0.0) + ff + fb
             tot_ms[name] = tot_ms.get(name, 0.0) + fms + bms
             log(f"conv {Cout}x{HW}x{Kc} {name}: fwd {fms:.3f} ms ({ff / fms / 1e9:.1f} TF/s), "

@@ -192,6 +192,19 @@ int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, co
 int LibShalom_sconv2d_forward(float *Y, const float *W, const float *X, long batch, long C, long H, long Wd,
                               long Cout, long R, long S, long stride_h, long stride_w, long pad_h, long pad_w,
                               long dil_h, long dil_w, int precision, LibShalom_stream_t stream);
+/* Backward of LibShalom_sconv2d_forward, both gradients in ONE launch, the
+ * im2col matrix again never stored: dX (batch x C x H x W) from dY
+ * (batch x Cout x P x Q) and W, and dW (Cout x C x R x S) = the sum over the
+ * batch, from dY and X.  Either output may be NULL to skip it.  dX needs
+ * stride 1 (LIBSHALOM_NOT_SUPPORTED otherwise; dW takes any stride): it is
+ * computed as the convolution of dY with the flipped, transposed filter,
+ * written by the library into its per-stream workspace.  Split reductions
+ * are summed inside the launch in a fixed order.  Limits as for the forward,
+ * with H + 2 pad_h (W + 2 pad_w) below 2^28 and one dY image below 2 GiB. */
+int LibShalom_sconv2d_backward(float *dX, float *dW, const float *dY, const float *W, const float *X, long batch,
+                               long C, long H, long Wd, long Cout, long R, long S, long stride_h, long stride_w,
+                               long pad_h, long pad_w, long dil_h, long dil_w, int precision,
+                               LibShalom_stream_t stream);
 /* P and Q of that convolution (LIBSHALOM_INVALID_VALUE for a bad geometry;
  * sizes <= 0 mean the dilated filter does not fit the padded input). */
 int LibShalom_conv2d_output_size(long H, long Wd, long R, long S, long stride_h, long stride_w, long pad_h,

@@ -87,6 +87,7 @@ SIGNATURES = {
     "LibShalom_sgemm_backward_strided": (_I, [_I, _I, _P, _P, _P, _P, _P, _L, _L, _L, _L, _L, _L, _L, _I, _P]),
     "LibShalom_sconv2d_forward": (_I, [_P, _P, _P] + [_L] * 13 + [_I, _P]),
     "LibShalom_conv2d_output_size": (_I, [_L] * 10 + [_LP, _LP]),
+    "LibShalom_sconv2d_backward": (_I, [_P] * 5 + [_L] * 13 + [_I, _P]),
     "LibShalom_autotune_batched": (_I, [_I, _I, _I, _P, _P, _P, _L, ct.POINTER(_I), _P]),
     "LibShalom_autotune_reset": (_I, []),
     "LibShalom_query_tuned_tile": (_I, [_I, _I, _I, _L, _L, _L, ct.POINTER(_I), ct.POINTER(_I)]),
@@ -283,6 +284,16 @@ def sconv2d_forward(Y, Wt, X, batch, C, H, W, Cout, R, S, stride=(1, 1), padding
     _check(_lib.LibShalom_sconv2d_forward(_ptr(Y), _ptr(Wt), _ptr(X), batch, C, H, W, Cout, R, S, stride[0],
                                           stride[1], padding[0], padding[1], dilation[0], dilation[1],
                                           precision, _stream(stream)), "LibShalom_sconv2d_forward")
+
+
+def sconv2d_backward(dX, dW, dY, Wt, X, batch, C, H, W, Cout, R, S, stride=(1, 1), padding=(0, 0),
+                     dilation=(1, 1), precision=FP32, stream=None):
+    """dX and dW of sconv2d_forward in one launch (either may be None to skip it; dX needs
+    stride 1); the im2col matrix is gathered, never stored."""
+    _check(_lib.LibShalom_sconv2d_backward(_ptr(dX), _ptr(dW), _ptr(dY), _ptr(Wt), _ptr(X), batch, C, H, W,
+                                           Cout, R, S, stride[0], stride[1], padding[0], padding[1],
+                                           dilation[0], dilation[1], precision, _stream(stream)),
+           "LibShalom_sconv2d_backward")
 
 
 def _arrays(Cs, As, Bs, Ms, Ns, Ks):

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -508,6 +509,86 @@ long conv_out_size(long in, long f, long stride, long pad, long dil)
 // ceil(2^32 / d) for the device's division by d (0 for d == 1).
 unsigned div_magic(long d) { return d == 1 ? 0u : (unsigned)((((unsigned long long)1 << 32) + d - 1) / d); }
 
+ConvGeom conv_geom(long C, long H, long W, long R, long S, long Q, long sh, long sw, long ph, long pw, long dh,
+                   long dw)
+{
+    ConvGeom g;
+    g.C = (int)C;
+    g.H = (int)H;
+    g.W = (int)W;
+    g.R = (int)R;
+    g.S = (int)S;
+    g.RS = (int)(R * S);
+    g.Q = (int)Q;
+    g.sh = (int)sh;
+    g.sw = (int)sw;
+    g.ph = (int)ph;
+    g.pw = (int)pw;
+    g.dh = (int)dh;
+    g.dw = (int)dw;
+    g.mRS = div_magic(R * S);
+    g.mS = div_magic(S);
+    g.mQ = div_magic(Q);
+    return g;
+}
+
+// Wf[c][o][R-1-r][S-1-s] = W[o][c][r][s]: the filter of the input gradient
+// of a stride-1 convolution, as a C x Cout*R*S matrix.
+__global__ void k_conv_flip(const float *__restrict__ Wt, float *__restrict__ Wf, int Cout, int C, int RS)
+{
+    const long n = (long)Cout * C * RS;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long o = i / ((long)C * RS), rem = i - o * C * RS;
+        const long c = rem / RS, t = rem - c * RS;
+        Wf[(c * Cout + o) * RS + (RS - 1 - t)] = Wt[i];
+    }
+}
+
+// Convolution backward as implicit GEMMs in one launch (either gradient may
+// be skipped with a null pointer).  dX, for stride 1 only: the convolution of
+// dY (Cout channels, P x Q) with the flipped, transposed filter, padding
+// dil*(f-1) - pad (negative padding crops), which gives back H x W.  dW: the
+// sum over images of dY_i times the transposed im2col matrix of X_i.
+int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *Wt, const float *X, long batch,
+                 long C, long H, long W, long Cout, long R, long S, long sh, long sw, long ph, long pw, long dh,
+                 long dw, long P, long Q, hipStream_t s)
+{
+    int dev;
+    DeviceInfo info;
+    int st = current_device(&dev, &info);
+    if (st) return st;
+    const long RS = R * S, K = C * RS, PQ = P * Q, HW = H * W;
+    const int kalign = (int)(32 / std::gcd(32L, RS) * RS);
+    BwdProblem X_, Y_;
+    std::memset(&X_, 0, sizeof(X_));
+    std::memset(&Y_, 0, sizeof(Y_));
+    ConvGeom gx = conv_geom(Cout, P, Q, R, S, W, 1, 1, dh * (R - 1) - ph, dw * (S - 1) - pw, dh, dw);
+    const ConvGeom gy = conv_geom(C, H, W, R, S, Q, sh, sw, ph, pw, dh, dw);
+    if (dX) {
+        void *wf;
+        st = workspace(dev, s, (size_t)Cout * K * sizeof(float), &wf);
+        if (st) return st;
+        const long n = Cout * K;
+        hipLaunchKernelGGL(k_conv_flip, dim3((unsigned)std::min(1024L, ceil_div(n, 256))), dim3(256), 0, s, Wt,
+                           (float *)wf, (int)Cout, (int)C, (int)RS);
+        if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
+        bwd_side(X_, wf, 0, (int)(Cout * RS), dY, Cout * PQ, 0, dX, C * HW, (int)HW, C, HW, Cout * RS, (int)batch);
+        X_.kalign = kalign;
+    }
+    if (dW) bwd_side(Y_, dY, Cout * PQ, (int)PQ, X, C * HW, 0, dW, 0, (int)K, Cout, K, PQ, (int)batch);
+    BwdProblem *probs[2];
+    int n = 0;
+    if (dX) probs[n++] = &X_;
+    if (dW) probs[n++] = &Y_;
+    long mind = LONG_MAX;
+    for (int i = 0; i < n; ++i) mind = std::min(mind, (long)std::min(probs[i]->R, probs[i]->Cc));
+    const int T = plan_problems(probs, n, mind >= 128, dev, info.num_cu, s);
+    if (T <= 0) return -T;
+    ConvBwdLaunch fn = bwd_table().conv_bwd[prec][T == 128];
+    if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    return fn(X_, Y_, gx, gy, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
+}
+
 // Convolution forward as an implicit GEMM over a batch of NCHW images: the
 // same sliced tile launch as a strided batch whose B operand is each
 // image's im2col matrix, gathered by the operand loads.  Slices of the
@@ -520,22 +601,7 @@ int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch
     int st = current_device(&dev, &info);
     if (st) return st;
     const long RS = R * S, K = C * RS, PQ = P * Q;
-    ConvGeom g;
-    g.H = (int)H;
-    g.W = (int)W;
-    g.R = (int)R;
-    g.S = (int)S;
-    g.RS = (int)RS;
-    g.Q = (int)Q;
-    g.sh = (int)sh;
-    g.sw = (int)sw;
-    g.ph = (int)ph;
-    g.pw = (int)pw;
-    g.dh = (int)dh;
-    g.dw = (int)dw;
-    g.mRS = div_magic(RS);
-    g.mS = div_magic(S);
-    g.mQ = div_magic(Q);
+    const ConvGeom g = conv_geom(C, H, W, R, S, Q, sh, sw, ph, pw, dh, dw);
     BwdProblem Pb;
     bwd_side(Pb, Wt, 0, (int)K, X, C * H * W, 0, Y, Cout * PQ, (int)PQ, Cout, PQ, K, (int)batch);
     Pb.kalign = (int)(32 / std::gcd(32L, RS) * RS);
@@ -1314,6 +1380,31 @@ int LibShalom_sconv2d_forward(float *Y, const float *Wt, const float *X, long ba
     if (!Y || !Wt || !X) return set_status(LIBSHALOM_INVALID_POINTER);
     return set_status(conv_fwd_run(kprec(precision), Y, Wt, X, batch, C, H, W, Cout, R, S, stride_h, stride_w, pad_h,
                                    pad_w, dil_h, dil_w, P, Q, (hipStream_t)stream));
+}
+
+int LibShalom_sconv2d_backward(float *dX, float *dWt, const float *dY, const float *Wt, const float *X, long batch,
+                               long C, long H, long W, long Cout, long R, long S, long stride_h, long stride_w,
+                               long pad_h, long pad_w, long dil_h, long dil_w, int precision,
+                               LibShalom_stream_t stream)
+{
+    if (!valid_precision(precision)) return set_status(LIBSHALOM_INVALID_VALUE);
+    if (batch < 0 || C <= 0 || H <= 0 || W <= 0 || Cout <= 0 || R <= 0 || S <= 0 || stride_h <= 0 ||
+        stride_w <= 0 || pad_h < 0 || pad_w < 0 || dil_h <= 0 || dil_w <= 0)
+        return set_status(LIBSHALOM_INVALID_VALUE);
+    const long P = conv_out_size(H, R, stride_h, pad_h, dil_h), Q = conv_out_size(W, S, stride_w, pad_w, dil_w);
+    if (P <= 0 || Q <= 0) return set_status(LIBSHALOM_INVALID_VALUE);
+    const double K = (double)C * R * S, img = (double)C * H * W, PQ = (double)P * Q;
+    if (batch > INT32_MAX || Cout > INT32_MAX || 4.0 * img >= 2147483648.0 || PQ >= 2147483648.0 ||
+        (K + R * S) * R * S >= 4294967296.0 || PQ * Q >= 4294967296.0 || H + 2.0 * pad_h >= 268435456.0 ||
+        W + 2.0 * pad_w >= 268435456.0 || 4.0 * Cout * PQ >= 2147483648.0 || K >= 2147483648.0 ||
+        // the input gradient's own gather: dY as the image, H x W as its output
+        ((double)Cout * R * S + R * S) * R * S >= 4294967296.0 || (double)H * W * W >= 4294967296.0)
+        return set_status(LIBSHALOM_NOT_SUPPORTED);
+    if (dX && (stride_h != 1 || stride_w != 1)) return set_status(LIBSHALOM_NOT_SUPPORTED);
+    if (batch == 0 || (!dX && !dWt)) return set_status(LIBSHALOM_SUCCESS);
+    if (!dY || (dX && !Wt) || (dWt && !X)) return set_status(LIBSHALOM_INVALID_POINTER);
+    return set_status(conv_bwd_run(kprec(precision), dX, dWt, dY, Wt, X, batch, C, H, W, Cout, R, S, stride_h,
+                                   stride_w, pad_h, pad_w, dil_h, dil_w, P, Q, (hipStream_t)stream));
 }
 
 int LibShalom_conv2d_output_size(long H, long W, long R, long S, long stride_h, long stride_w, long pad_h,

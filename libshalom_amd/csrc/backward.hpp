@@ -43,13 +43,16 @@ struct BwdProblem {
 
 // The segments of one unit's slice as a job stream (see Tile::stream).  The
 // problem's fields are copied in (scalar registers): a pointer to the kernel
-// argument would send it through private memory.  With IM2COL_ the right
-// operand of every image is the im2col matrix of the convolution `geom`
-// over the image Rt + img * sR: the job's B is that image at the slice's
-// first channel and its ldb the tile's first output pixel.
-template <class Cfg, bool IM2COL_ = false>
+// argument would send it through private memory.  With IM2COL_ = kIm2col
+// the right operand of every image is the im2col matrix of the convolution
+// `geom` over the image Rt + img * sR: the job's B is that image at the
+// slice's first channel and its ldb the tile's first output pixel.  With
+// kIm2colT the right operand is stored transposed and is that im2col
+// matrix (rows = taps from c0, reduction = output pixels from rho0); the
+// job's B is the image.
+template <class Cfg, int IM2COL_ = kPlainB>
 struct SliceSrc {
-    static constexpr bool IM2COL = IM2COL_;
+    static constexpr int IM2COL = IM2COL_;
     const void *L, *Rt;
     long long sL, sR;
     int lda, ldb;
@@ -77,7 +80,7 @@ struct SliceSrc {
         d.N = min(Cfg::BN, Cc - c0);
         d.K = rho1 - rho0;
         d.lda = lda;
-        d.ldb = IM2COL ? c0 : ldb;
+        d.ldb = IM2COL == kIm2col ? c0 : ldb;
         d.ldc = ldo;
         d.flags = j + 1 < nseg ? kChain : 0;
         return d;
@@ -114,7 +117,7 @@ __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_o
     if (threadIdx.x == 0) P.counters[tg] = 0;  // ready for the next launch
 }
 
-template <class Cfg, bool IM2COL = false>
+template <class Cfg, int IM2COL = kPlainB>
 __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, int *flag, const ConvGeom &geom = {})
 {
     const int tg = u / P.nslices, s = u - tg * P.nslices;
@@ -152,8 +155,10 @@ __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, 
         src.rho1 = P.red;
     }
     // a convolution slice starts on a channel (slices are multiples of R*S)
-    if constexpr (IM2COL)
+    if constexpr (IM2COL == kIm2col)
         src.boff = (long long)(src.rho0 / geom.RS) * geom.H * geom.W;
+    else if constexpr (IM2COL == kIm2colT)
+        src.boff = 0;
     else
         src.boff = Cfg::TB ? (long long)src.c0 * P.ldb + src.rho0 : (long long)src.rho0 * P.ldb + src.c0;
     if (P.nslices == 1) {
@@ -209,7 +214,27 @@ k_conv_fwd(BwdProblem P, ConvGeom g)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
     __shared__ int flag;
-    bwd_unit<Cfg, true>(P, blockIdx.x, lds, &flag, g);
+    bwd_unit<Cfg, kIm2col>(P, blockIdx.x, lds, &flag, g);
+}
+
+// Convolution backward as two implicit GEMMs in ONE launch.  X units: the
+// input gradient of a stride-1 convolution, itself a convolution of dY
+// (geometry gx) with the filter flipped and transposed (C x Cout*R*S, in
+// the problem's left operand).  Y units: the filter gradient
+// dW (Cout x C*R*S) = sum over images of dY_i (Cout x P*Q) times the
+// transposed im2col matrix of X_i, gathered from the image (geometry gy),
+// the sum over images and pixels cut into slices as for a shared operand.
+template <class CX, class CY>
+__global__ __attribute__((amdgpu_flat_work_group_size(CX::NT, CX::NT))) void
+k_conv_backward(BwdProblem X, BwdProblem Y, ConvGeom gx, ConvGeom gy)
+{
+    __shared__ __attribute__((aligned(16))) char lds[BwdLds<CX, CY>::BYTES];
+    __shared__ int flag;
+    const int u = blockIdx.x;
+    if (u < X.units)
+        bwd_unit<CX, kIm2col>(X, u, lds, &flag, gx);
+    else
+        bwd_unit<CY, kIm2colT>(Y, u - X.units, lds, &flag, gy);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,6 +243,21 @@ k_conv_fwd(BwdProblem P, ConvGeom g)
 typedef hipError_t (*BwdLaunch)(const BwdProblem &X, const BwdProblem &Y, hipStream_t s);
 typedef hipError_t (*StridedLaunch)(const BwdProblem &P, hipStream_t s);
 typedef hipError_t (*ConvLaunch)(const BwdProblem &P, const ConvGeom &g, hipStream_t s);
+typedef hipError_t (*ConvBwdLaunch)(const BwdProblem &X, const BwdProblem &Y, const ConvGeom &gx,
+                                    const ConvGeom &gy, hipStream_t s);
+
+template <int PREC, int BT>
+hipError_t launch_conv_backward(const BwdProblem &X, const BwdProblem &Y, const ConvGeom &gx, const ConvGeom &gy,
+                                hipStream_t s)
+{
+    typedef TileCfg<PREC, 0, 0, BT, BT> CX;  // dX: flipped filter x im2col(dY)
+    typedef TileCfg<PREC, 0, 1, BT, BT> CY;  // dW: dY x im2col(X)^T
+    static_assert(CX::NT == CY::NT, "one workgroup size for both problems");
+    const int grid = X.units + Y.units;
+    if (grid <= 0) return hipSuccess;
+    hipLaunchKernelGGL((k_conv_backward<CX, CY>), dim3(grid), dim3(CX::NT), 0, s, X, Y, gx, gy);
+    return hipGetLastError();
+}
 
 template <int PREC, int BT>
 hipError_t launch_conv_fwd(const BwdProblem &P, const ConvGeom &g, hipStream_t s)
@@ -259,6 +299,7 @@ struct BwdTable {
     BwdLaunch fn[NPREC][2][2][NBWD_TILE];
     StridedLaunch strided[NPREC][2][2][NBWD_TILE];
     ConvLaunch conv_fwd[NPREC][NBWD_TILE];
+    ConvBwdLaunch conv_bwd[NPREC][NBWD_TILE];
 };
 BwdTable &bwd_table();
 
@@ -283,6 +324,8 @@ void register_backward(BwdTable &t)
     t.strided[PREC][1][1][1] = &launch_strided<PREC, 1, 1, 128>;
     t.conv_fwd[PREC][0] = &launch_conv_fwd<PREC, 64>;
     t.conv_fwd[PREC][1] = &launch_conv_fwd<PREC, 128>;
+    t.conv_bwd[PREC][0] = &launch_conv_backward<PREC, 64>;
+    t.conv_bwd[PREC][1] = &launch_conv_backward<PREC, 128>;
 }
 
 void register_backward_fp32(BwdTable &t);

@@ -63,12 +63,18 @@ struct GemmDesc {
 // (c, r, s), or 0 where that falls in the padding.  The im2col matrix of an
 // image is (C*R*S) x (P*Q): row t = (c*R + r)*S + s, column n = p*Q + q.
 struct ConvGeom {
-    int H, W;               // input height, width
+    int C, H, W;            // input channels, height, width
     int R, S, RS;           // filter height, width; R x S
     int Q;                  // output width
     int sh, sw, ph, pw, dh, dw;  // stride, padding, dilation
     unsigned mRS, mS, mQ;   // ceil(2^32 / d) for d = RS, S, Q; 0 when d == 1
 };
+
+// How a job source's B operand is formed: read from memory; gathered as a
+// convolution's im2col matrix (k = filter tap, column = output pixel); or,
+// for a B stored transposed, gathered as the im2col matrix itself (stored
+// row = filter tap, k = output pixel).
+enum BSource : int { kPlainB = 0, kIm2col = 1, kIm2colT = 2 };
 
 // x / d for the multiplier m = ceil(2^32 / d) (m == 0: d == 1).  Exact when
 // x * d < 2^32: with e = m*d - 2^32 < d, x*m / 2^32 = x/d + x*e / (d 2^32),
@@ -476,6 +482,74 @@ struct Stage {
             for (int e = 0; e < VEC; ++e) v[i][e] = buf_load1<T>(rx, off[i][e]);
     }
 
+    // Per-lane filter taps of the rows this thread loads from a transposed
+    // im2col panel (row = tap t0 + row, k = output pixel), fixed for one
+    // output tile: the tap's byte offset (c*H*W + r*dh*W + s*dw) * 4 and its
+    // row / column offsets r*dh, s*dw.  A row past the tile (or the panel)
+    // gets dy = 2^29, which keeps every pixel out of the image.
+    struct Taps {
+        int toff[PER], dy[PER], dx[PER];
+    };
+
+    __device__ __forceinline__ static Taps taps(const ConvGeom &g, int t0, int N, int tid)
+    {
+        static_assert(KCONTIG && !LANEROW, "the transposed im2col panel is k-contiguous");
+        Taps tp;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int idx = tid + i * NT;
+            int row, k;
+            coords(idx, row, k);
+            const bool act = (NVEC % NT == 0) | (idx < NVEC);
+            const int t = t0 + row;
+            const int c = udiv(t, g.mRS), rem = t - c * g.RS;
+            const int r = udiv(rem, g.mS), s = rem - r * g.S;
+            tp.dy[i] = (act & (row < N)) ? r * g.dh : (1 << 29);
+            tp.dx[i] = s * g.dw;
+            tp.toff[i] = (int)(((unsigned)c * (unsigned)(g.H * g.W) + (unsigned)(r * g.dh * g.W + s * g.dw)) * ES);
+        }
+        return tp;
+    }
+
+    // The chunk at k0 of a transposed im2col panel: element (row, k) is
+    // X[c][p*sh - ph + r*dh][q*sw - pw + s*dw] for the row's tap (c, r, s)
+    // and output pixel n0 + k0 + k = p*Q + q, or 0 in the padding and past
+    // kend pixels.  Each thread holds VEC consecutive pixels of one tap:
+    // the first is decoded (one division), the others follow by stepping
+    // along the output row.
+    __device__ __forceinline__ void load_im2col_t(const T *__restrict__ X, const ConvGeom &g, const Taps &tp, int n0,
+                                                  int kend, int k0, bool live, int tid)
+    {
+        static_assert(KCONTIG && !LANEROW, "the transposed im2col panel is k-contiguous");
+        const int kc = min(kend - k0, BK);
+        kcut = kc;
+        const int HW = g.H * g.W;
+        const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0) ? g.C * HW * ES : 0);
+        const unsigned step_col = g.sw * ES, step_row = (unsigned)(g.sh * g.W - (g.Q - 1) * g.sw) * ES;
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            int row, k;
+            coords(tid + i * NT, row, k);
+            const int n = n0 + k0 + k;
+            int p = udiv(n, g.mQ), q = n - p * g.Q;
+            int ih = p * g.sh - g.ph, iw = q * g.sw - g.pw;
+            // byte offset of the pixel's tap (0, 0) in channel 0, modulo 2^32
+            unsigned pix = ((unsigned)ih * (unsigned)g.W + (unsigned)iw) * ES;
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                const bool ok = (k + e < kc) & ((unsigned)(ih + tp.dy[i]) < (unsigned)g.H) &
+                                ((unsigned)(iw + tp.dx[i]) < (unsigned)g.W);
+                v[i][e] = buf_load1<T>(rx, ok ? (int)((unsigned)tp.toff[i] + pix) : kOOB);
+                // next output pixel
+                const bool wrap = q + 1 == g.Q;
+                q = wrap ? 0 : q + 1;
+                ih += wrap ? g.sh : 0;
+                iw = wrap ? -g.pw : iw + g.sw;
+                pix += wrap ? step_row : step_col;
+            }
+        }
+    }
+
     // Registers -> LDS panel.  In a k-contiguous panel the elements of a
     // group at k >= kcut (in range: the next columns of the row) are zeroed
     // here, after the loads have landed, and only in a chunk that ends
@@ -541,10 +615,25 @@ struct Tile {
     static constexpr int WM = Cfg::WM, WN = Cfg::WN;
     typedef Stage<T, Cfg::NP, BM, BK, Cfg::A_KCONTIG, Cfg::NT> StageA;
     typedef Stage<T, Cfg::NP, BN, BK, Cfg::B_KCONTIG, Cfg::NT> StageB;
-    // B stage of an im2col source (lane-per-row loads in every mode)
+    // B stage of an im2col source (lane-per-row loads in every mode); the
+    // transposed im2col operand uses the ordinary k-contiguous stage
     typedef Stage<T, Cfg::NP, BN, BK, false, Cfg::NT, true> StageBI;
     template <class Src>
-    using StageBFor = typename std::conditional<Src::IM2COL, StageBI, StageB>::type;
+    using StageBFor = typename std::conditional<Src::IM2COL == kIm2col, StageBI, StageB>::type;
+
+    struct NoLanes {};
+    // The per-lane coordinates a computed B operand keeps for the stream's
+    // output tile (the same for every job of such a stream).
+    template <class Src>
+    __device__ __forceinline__ static auto lanes(const Src &src, const GemmDesc &d0, int tid)
+    {
+        if constexpr (Src::IM2COL == kIm2col)
+            return StageBI::pixels(src.geom, d0.ldb, d0.N, tid);
+        else if constexpr (Src::IM2COL == kIm2colT)
+            return StageB::taps(src.geom, src.c0, d0.N, tid);
+        else
+            return NoLanes{};
+    }
 
     Acc acc[WM][WN];
     // Result scaling: with `scale` set, stores write alpha * product + beta *
@@ -690,18 +779,22 @@ struct Tile {
 
     // Global -> registers: the K chunk at k0 of job d (live == false: a
     // placeholder load that returns zeros, keeping instruction counts static).
-    // A source with IM2COL set describes B as a convolution's im2col matrix:
-    // d.B is the image at the job's first channel and d.ldb the tile's first
-    // output pixel; every job of such a stream has the same output tile,
-    // whose per-lane pixel coordinates px are computed once.
-    template <class Src, class SB>
-    __device__ __forceinline__ static void load_chunk(StageA &sa, SB &sb, const Src &src,
-                                                      const typename SB::Pixels &px, const GemmDesc &d, int k0,
-                                                      bool live, int tid)
+    // A source with IM2COL == kIm2col describes B as a convolution's im2col
+    // matrix: d.B is the image at the job's first channel and d.ldb the
+    // tile's first output pixel.  With kIm2colT, B is stored transposed and
+    // is the im2col matrix itself (rows = taps from src.c0, k = output
+    // pixels from src.rho0); d.B is the image.  Every job of such a stream
+    // has the same output tile, whose per-lane coordinates px are computed
+    // once.
+    template <class Src, class SB, class PX>
+    __device__ __forceinline__ static void load_chunk(StageA &sa, SB &sb, const Src &src, const PX &px,
+                                                      const GemmDesc &d, int k0, bool live, int tid)
     {
         sa.load((const T *)d.A, d.lda, d.M, d.K, k0, live, tid);
-        if constexpr (Src::IM2COL)
+        if constexpr (Src::IM2COL == kIm2col)
             sb.load_im2col((const T *)d.B, src.geom, px, d.K, k0, live, tid);
+        else if constexpr (Src::IM2COL == kIm2colT)
+            sb.load_im2col_t((const T *)d.B, src.geom, px, src.rho0, d.K, k0, live, tid);
         else
             sb.load((const T *)d.B, d.ldb, d.N, d.K, k0, live, tid);
     }
@@ -756,9 +849,9 @@ struct Tile {
     // One step of the chunk pipeline.  `cur` is in LDS buffer bi; its
     // successor n1 is in flight into stage (a1, b1); stage (a0, b0) is free and
     // receives n2.  Two chunks stay in flight while `cur` is multiplied.
-    template <class Src, class SB>
+    template <class Src, class SB, class PX>
     __device__ __forceinline__ bool step(Cursor &cur, Cursor &n1, Cursor &n2, StageA &a0, SB &b0, StageA &a1,
-                                         SB &b1, const typename SB::Pixels &px, char *lds, int bi, const Src &src,
+                                         SB &b1, const PX &px, char *lds, int bi, const Src &src,
                                          int stride, int tid,
                                          int lane, int wave)
     {
@@ -814,8 +907,7 @@ struct Tile {
         advance(n2, src, stride);
         StageA a0, a1;
         StageBFor<Src> b0, b1;
-        typename StageBFor<Src>::Pixels px;
-        if constexpr (Src::IM2COL) px = StageBFor<Src>::pixels(src.geom, d0.ldb, d0.N, tid);
+        const auto px = lanes(src, d0, tid);
         load_chunk(a0, b0, src, px, d0, 0, true, tid);
         load_chunk(a1, b1, src, px, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
         begin(d0.M, d0.N, 0, 0, wave);
@@ -880,7 +972,7 @@ struct Tile {
 
 // Job source over a device array of jobs.
 struct ArraySrc {
-    static constexpr bool IM2COL = false;
+    static constexpr int IM2COL = kPlainB;
     const GemmDesc *__restrict__ jobs;
     int n;
     __device__ __forceinline__ int count() const { return n; }
@@ -900,7 +992,7 @@ struct ArraySrc {
 
 // Job source of exactly one job held in registers.
 struct OneSrc {
-    static constexpr bool IM2COL = false;
+    static constexpr int IM2COL = kPlainB;
     GemmDesc job;
     __device__ __forceinline__ int count() const { return 1; }
     __device__ __forceinline__ GemmDesc get(int) const { return job; }

@@ -61,6 +61,7 @@ def lib():
                                       ct.c_double, _dp]
         L.oracle_check_kv.restype = _L
         L.oracle_sconv2d_ref.argtypes = [_fp, _fp, _dp, _dp] + [_L] * 15
+        L.oracle_sconv2d_bwd_ref.argtypes = [_fp, _fp, _fp, _dp, _dp, _dp, _dp] + [_L] * 15
         L.cpu_sgemm.argtypes = [ct.c_int, ct.c_int, _fp, _fp, _fp, _L, _L, _L]
         L.cpu_dgemm.argtypes = [ct.c_int, ct.c_int, _dp, _dp, _dp, _L, _L, _L]
         L.cpu_sgemm_batched.argtypes = [ct.c_int, ct.c_int, _L, ct.c_void_p, ct.c_void_p,
@@ -164,6 +165,27 @@ def sconv2d_ref(X, Wt, stride=(1, 1), padding=(0, 0), dilation=(1, 1)):
     lib().oracle_sconv2d_ref(_p(X, _fp), _p(Wt, _fp), _p(Y, _dp), _p(A, _dp), batch, C, H, W, Cout, R, S,
                              stride[0], stride[1], padding[0], padding[1], dilation[0], dilation[1], P, Q)
     return Y, A
+
+
+def sconv2d_bwd_ref(X, Wt, dY, stride=(1, 1), padding=(0, 0), dilation=(1, 1)):
+    """Long-double gradients of the convolution for the loss sum(Y * dY).
+    Returns (dX, |dX| bound sums, dW, |dW| bound sums), float64, shaped like X and Wt."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Wt = np.ascontiguousarray(Wt, dtype=np.float32)
+    dY = np.ascontiguousarray(dY, dtype=np.float32)
+    batch, C, H, W = X.shape
+    Cout, _, R, S = Wt.shape
+    P = conv_out(H, R, stride[0], padding[0], dilation[0])
+    Q = conv_out(W, S, stride[1], padding[1], dilation[1])
+    assert dY.shape == (batch, Cout, P, Q)
+    dX = np.empty(X.shape, dtype=np.float64)
+    aX = np.empty_like(dX)
+    dW = np.empty(Wt.shape, dtype=np.float64)
+    aW = np.empty_like(dW)
+    lib().oracle_sconv2d_bwd_ref(_p(X, _fp), _p(Wt, _fp), _p(dY, _fp), _p(dX, _dp), _p(aX, _dp), _p(dW, _dp),
+                                 _p(aW, _dp), batch, C, H, W, Cout, R, S, stride[0], stride[1], padding[0],
+                                 padding[1], dilation[0], dilation[1], P, Q)
+    return dX, aX, dW, aW
 
 
 def check(C, Cref, absAB, K, u=U_F32):

@@ -6,6 +6,7 @@
  */
 #include "oracle.h"
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #define LCG_A 0x5DEECE66DULL
@@ -163,6 +164,52 @@ void oracle_sconv2d_ref(const float *X, const float *Wt, double *Yref, double *a
                     Yref[i] = (double)acc;
                     absWX[i] = (double)aa;
                 }
+}
+
+/* Backward of oracle_sconv2d_ref for the loss sum(Y * dY):
+ *   dX[b][c][ih][iw] = sum over (o, r, s, p, q) that read X[b][c][ih][iw] of Wt[o][c][r][s] * dY[b][o][p][q]
+ *   dW[o][c][r][s]   = sum over (b, p, q) of dY[b][o][p][q] * X[b][c][ih][iw]
+ * (ih = p*sh - ph + r*dh, iw = q*sw - pw + s*dw inside the image): every
+ * product of the forward sum, scattered to the operand it came from.  Long
+ * double sums; absdX / absdW get the sums of |.| products. */
+void oracle_sconv2d_bwd_ref(const float *X, const float *Wt, const float *dY, double *dXref, double *absdX,
+                            double *dWref, double *absdW, long batch, long C, long H, long W, long Cout, long R,
+                            long S, long sh, long sw, long ph, long pw, long dh, long dw, long P, long Q)
+{
+    const long nx = batch * C * H * W, nw = Cout * C * R * S;
+    long double *gx = (long double *)calloc((size_t)(2 * nx + 2 * nw), sizeof(long double));
+    if (!gx) return;
+    long double *ax = gx + nx, *gw = ax + nx, *aw = gw + nw;
+    for (long b = 0; b < batch; ++b)
+        for (long o = 0; o < Cout; ++o)
+            for (long p = 0; p < P; ++p)
+                for (long q = 0; q < Q; ++q) {
+                    const long double g = dY[((b * Cout + o) * P + p) * Q + q];
+                    for (long c = 0; c < C; ++c)
+                        for (long r = 0; r < R; ++r) {
+                            const long ih = p * sh - ph + r * dh;
+                            if (ih < 0 || ih >= H) continue;
+                            for (long s = 0; s < S; ++s) {
+                                const long iw = q * sw - pw + s * dw;
+                                if (iw < 0 || iw >= W) continue;
+                                const long xi = ((b * C + c) * H + ih) * W + iw, wi = ((o * C + c) * R + r) * S + s;
+                                const long double x = X[xi], w = Wt[wi];
+                                gx[xi] += w * g;
+                                ax[xi] += fabsl(w) * fabsl(g);
+                                gw[wi] += g * x;
+                                aw[wi] += fabsl(g) * fabsl(x);
+                            }
+                        }
+                }
+    for (long i = 0; i < nx; ++i) {
+        dXref[i] = (double)gx[i];
+        absdX[i] = (double)ax[i];
+    }
+    for (long i = 0; i < nw; ++i) {
+        dWref[i] = (double)gw[i];
+        absdW[i] = (double)aw[i];
+    }
+    free(gx);
 }
 
 static long check_core2(const void *Cv, int is_f64, const double *Cref, const double *absAB,

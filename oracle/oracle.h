@@ -31,6 +31,9 @@
  *     conv2d, against torch's unfold (im2col) followed by oracle_sgemm_ref,
  *     and by closed forms (a one-tap filter shifts the input, all-ones
  *     operands count the taps inside the image) in tests/test_oracle.py.
+ *   - oracle_sconv2d_bwd_ref: its gradients; pinned against torch's float64
+ *     autograd of conv2d and by the adjoint identities
+ *     <conv(X, W), dY> = <X, dX> = <W, dW> in tests/test_oracle.py.
  *   - cpu_sgemm / cpu_dgemm (+ _batched): the reference's C row-major
  *     verification loop as a triple loop (i-k-j for NN, i-j-k dot for NT),
  *     OpenMP over rows or over batch entries.  Used only as the reported
@@ -79,6 +82,12 @@ void oracle_dgemm_ref_flat(int transa, int transb, long count, const double *A, 
 void oracle_sconv2d_ref(const float *X, const float *Wt, double *Yref, double *absWX, long batch, long C, long H,
                         long W, long Cout, long R, long S, long sh, long sw, long ph, long pw, long dh, long dw,
                         long P, long Q);
+/* Its backward for the loss sum(Y * dY): dX (like X) and dW (like Wt), each
+ * forward product scattered to the operand it came from; long double sums
+ * and the sums of |.| products for the bound. */
+void oracle_sconv2d_bwd_ref(const float *X, const float *Wt, const float *dY, double *dXref, double *absdX,
+                            double *dWref, double *absdW, long batch, long C, long H, long W, long Cout, long R,
+                            long S, long sh, long sw, long ph, long pw, long dh, long dw, long P, long Q);
 /* Elementwise check against the reference.
  * stats[0] = max |C - Cref| / (2*K*u*absAB)   (<= 1 means within tolerance)
  * stats[1] = max |C - Cref| / absAB            (componentwise relative error)

@@ -139,3 +139,92 @@ def test_conv_empty_batch_is_a_no_op():
     ls.sconv2d_forward(y, y, y, 0, 3, 8, 8, 4, 3, 3, (1, 1), (1, 1), (1, 1), ls.FP32)
     torch.cuda.synchronize()
     assert (y.cpu() == 3.0).all()
+
+
+# Backward: stride-1 geometries (dX and dW) and strided ones (dW only).
+BWD_GEOMS = [g for g in GEOMS if g[7] == (1, 1)]
+BWD_STRIDED = [g for g in GEOMS if g[7] != (1, 1)]
+
+
+def _run_bwd(X, Wt, dY, st, pad, dil, prec, want_dx=True, guard=64):
+    batch, C, H, W = X.shape
+    Cout, _, R, S = Wt.shape
+    nx, nw = X.size, Wt.size
+    dX = torch.full((nx + guard,), float("nan"), device="cuda") if want_dx else None
+    dW = torch.full((nw + guard,), float("nan"), device="cuda")
+    ls.sconv2d_backward(dX, dW, torch.from_numpy(dY).cuda(), torch.from_numpy(Wt).cuda(),
+                        torch.from_numpy(X).cuda(), batch, C, H, W, Cout, R, S, st, pad, dil, prec)
+    torch.cuda.synchronize()
+    gx = (dX[:nx].reshape(X.shape).cpu().numpy(), dX[nx:].cpu().numpy()) if want_dx else (None, None)
+    return gx, (dW[:nw].reshape(Wt.shape).cpu().numpy(), dW[nw:].cpu().numpy())
+
+
+def _bwd_case(g, prec, want_dx):
+    batch, C, H, W, Cout, R, S, st, pad, dil = g
+    X, Wt = _inputs(batch, C, H, W, Cout, R, S, seed=sum(g[:7]) + 1)
+    P, Q = ls.conv2d_output_size(H, W, R, S, st, pad, dil)
+    dY = np.random.default_rng(sum(g[:7])).uniform(-1, 1, (batch, Cout, P, Q)).astype(np.float32)
+    (gx, tx), (gw, tw) = _run_bwd(X, Wt, dY, st, pad, dil, prec, want_dx)
+    rx, ax, rw, aw = oracle.sconv2d_bwd_ref(X, Wt, dY, st, pad, dil)
+    _assert_within(gw, rw, aw, batch * P * Q, prec, f"dW {g}")
+    assert np.isnan(tw).all(), "write past the end of dW"
+    if want_dx:
+        _assert_within(gx, rx, ax, Cout * R * S, prec, f"dX {g}")
+        assert np.isnan(tx).all(), "write past the end of dX"
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("g", BWD_GEOMS)
+def test_conv_backward_matches_oracle(g, prec):
+    print(f"conv backward {g} precision {prec}", flush=True)
+    _bwd_case(g, prec, True)
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+@pytest.mark.parametrize("g", BWD_STRIDED)
+def test_conv_backward_strided_filter_gradient(g, prec):
+    print(f"conv backward (dW only) {g} precision {prec}", flush=True)
+    _bwd_case(g, prec, False)
+
+
+def _torch_grads(X, Wt, dY):
+    x = torch.from_numpy(X).double().requires_grad_()
+    w = torch.from_numpy(Wt).double().requires_grad_()
+    torch.nn.functional.conv2d(x, w, padding=1).backward(torch.from_numpy(dY).double())
+    xa = torch.from_numpy(X).double().abs().requires_grad_()
+    wa = torch.from_numpy(Wt).double().abs().requires_grad_()
+    torch.nn.functional.conv2d(xa, wa, padding=1).backward(torch.from_numpy(dY).double().abs())
+    return x.grad.numpy(), xa.grad.numpy(), w.grad.numpy(), wa.grad.numpy()
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+@pytest.mark.parametrize("layer", [(64, 64, 56, 2), (128, 128, 28, 2), (256, 256, 14, 2), (512, 512, 7, 2),
+                                   (256, 16, 32, 16)])
+def test_conv_backward_layers_match_fp64(layer, prec):
+    """The 3 x 3 layers against torch's float64 autograd (the |.| products
+    for the bound are the gradients of the |.| problem)."""
+    Cout, C, HW, batch = layer
+    print(f"conv backward layer {layer} precision {prec}", flush=True)
+    X, Wt = _inputs(batch, C, HW, HW, Cout, 3, 3, seed=Cout + C + 1)
+    dY = np.random.default_rng(Cout).uniform(-1, 1, (batch, Cout, HW, HW)).astype(np.float32)
+    (gx, _), (gw, _) = _run_bwd(X, Wt, dY, (1, 1), (1, 1), (1, 1), prec)
+    rx, ax, rw, aw = _torch_grads(X, Wt, dY)
+    _assert_within(gx, rx, ax, Cout * 9, prec, f"dX {layer}")
+    _assert_within(gw, rw, aw, batch * HW * HW, prec, f"dW {layer}")
+
+
+def test_conv_backward_deterministic_and_single_gradients():
+    """Sliced reductions summed in a fixed order: the same bits on every
+    call, and each gradient alone equals the fused launch's."""
+    X, Wt = _inputs(4, 128, 14, 14, 64, 3, 3, seed=21)
+    dY = np.random.default_rng(5).uniform(-1, 1, (4, 64, 14, 14)).astype(np.float32)
+    (x1, _), (w1, _) = _run_bwd(X, Wt, dY, (1, 1), (1, 1), (1, 1), ls.BF16X3)
+    (x2, _), (w2, _) = _run_bwd(X, Wt, dY, (1, 1), (1, 1), (1, 1), ls.BF16X3)
+    assert np.array_equal(x1, x2) and np.array_equal(w1, w2)
+    (_, _), (w3, _) = _run_bwd(X, Wt, dY, (1, 1), (1, 1), (1, 1), ls.BF16X3, want_dx=False)
+    assert np.isfinite(w3).all()
+    dX = torch.full(X.shape, float("nan"), device="cuda")
+    ls.sconv2d_backward(dX, None, torch.from_numpy(dY).cuda(), torch.from_numpy(Wt).cuda(), None, 4, 128, 14, 14,
+                        64, 3, 3, (1, 1), (1, 1), (1, 1), ls.BF16X3)
+    torch.cuda.synchronize()
+    assert np.isfinite(dX.cpu().numpy()).all()

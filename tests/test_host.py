@@ -318,3 +318,19 @@ def test_conv_output_size_matches_torch(H, R, st, pad, dil):
         return
     y = torch.nn.functional.conv2d(x, w, stride=st, padding=pad, dilation=dil)
     assert (P, Q) == tuple(y.shape[2:])
+
+
+def test_conv_backward_argument_errors_before_device():
+    L = ls.lib()
+    g = (1, 3, 8, 8, 4, 3, 3)
+    # input gradient of a strided convolution: not supported
+    assert L.LibShalom_sconv2d_backward(0x1000, 0x1000, 0x1000, 0x1000, 0x1000, *g, 2, 2, 1, 1, 1, 1, 0, None) == 5
+    assert L.LibShalom_sconv2d_backward(0x1000, 0x1000, None, 0x1000, 0x1000, *g, 1, 1, 1, 1, 1, 1, 0, None) == 2
+    assert L.LibShalom_sconv2d_backward(0x1000, None, 0x1000, None, 0x1000, *g, 1, 1, 1, 1, 1, 1, 0, None) == 2
+    assert L.LibShalom_sconv2d_backward(None, 0x1000, 0x1000, 0x1000, None, *g, 1, 1, 1, 1, 1, 1, 0, None) == 2
+    assert L.LibShalom_sconv2d_backward(None, None, 0x1000, 0x1000, 0x1000, *g, 1, 1, 1, 1, 1, 1, 0, None) == 0
+    assert L.LibShalom_sconv2d_backward(0x1000, 0x1000, 0x1000, 0x1000, 0x1000, *g, 1, 1, 1, 1, 1, 1, 9, None) == 1
+    assert L.LibShalom_sconv2d_backward(0x1000, 0x1000, 0x1000, 0x1000, 0x1000, 1, 3, 2, 8, 4, 5, 3,
+                                        1, 1, 0, 1, 1, 1, 0, None) == 1  # P = 0
+    assert L.LibShalom_sconv2d_backward(0x1000, 0x1000, 0x1000, 0x1000, 0x1000, 0, 3, 8, 8, 4, 3, 3,
+                                        1, 1, 1, 1, 1, 1, 0, None) == 0  # empty batch

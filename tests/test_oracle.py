@@ -13,6 +13,8 @@
   the reference GEMM (bit-identical), and by closed forms: a one-tap filter
   copies or shifts the input, all-ones operands count the taps inside the
   image.  A filter flipped along one axis must be caught.
+* Convolution gradients: against torch's float64 autograd, and the adjoint
+  identities <conv(X, W), dY> = <X, dX> = <W, dW>.
 """
 import json
 import os
@@ -280,3 +282,32 @@ def test_conv_reference_catches_flipped_filter():
     K = g[1] * g[5] * g[6]
     assert oracle.check(Y.astype(np.float32), Y, A, K)[0] == 0
     assert oracle.check(Yf.astype(np.float32), Y, A, K)[0] > 0
+
+
+@pytest.mark.parametrize("g", CONV_GEOMS)
+def test_conv_backward_reference_matches_torch_autograd(g):
+    torch = pytest.importorskip("torch")
+    X, Wt = _conv_inputs(g, 13)
+    st, pad, dil = g[7:]
+    Y, _ = oracle.sconv2d_ref(X, Wt, st, pad, dil)
+    dY = np.random.default_rng(3).uniform(-1, 1, Y.shape).astype(np.float32)
+    dX, aX, dW, aW = oracle.sconv2d_bwd_ref(X, Wt, dY, st, pad, dil)
+    x = torch.from_numpy(X).double().requires_grad_()
+    w = torch.from_numpy(Wt).double().requires_grad_()
+    y = torch.nn.functional.conv2d(x, w, stride=st, padding=pad, dilation=dil)
+    y.backward(torch.from_numpy(dY).double())
+    np.testing.assert_allclose(dX, x.grad.numpy(), rtol=0, atol=1e-12)
+    np.testing.assert_allclose(dW, w.grad.numpy(), rtol=0, atol=1e-12)
+    assert (aX >= np.abs(dX) - 1e-12).all() and (aW >= np.abs(dW) - 1e-12).all()
+
+
+@pytest.mark.parametrize("g", CONV_GEOMS[:3])
+def test_conv_backward_reference_adjoint_identities(g):
+    X, Wt = _conv_inputs(g, 17)
+    st, pad, dil = g[7:]
+    Y, _ = oracle.sconv2d_ref(X, Wt, st, pad, dil)
+    dY = np.random.default_rng(4).uniform(-1, 1, Y.shape).astype(np.float32)
+    dX, _, dW, _ = oracle.sconv2d_bwd_ref(X, Wt, dY, st, pad, dil)
+    lhs = float(np.sum(Y * dY.astype(np.float64)))
+    assert abs(lhs - float(np.sum(X.astype(np.float64) * dX))) < 1e-10
+    assert abs(lhs - float(np.sum(Wt.astype(np.float64) * dW))) < 1e-10
