@@ -9,7 +9,11 @@ in one launch).  Its gradients are checked against
   * the gradient torch's autograd gives for torch.matmul in float64.
 
 An im2col-convolution variant (weights shared by a batch of 32 images) uses a
-batched plan forward and LibShalom_sgemm_backward_strided backward.
+batched plan forward and LibShalom_sgemm_backward_strided backward.  The
+implicit-GEMM convolution (LibShalom_sconv2d_forward / _backward, on NCHW
+images, no im2col matrix) is checked the same way: central differences of
+sum(G * conv(X, W)) evaluated by the oracle's long-double convolution, and
+torch's float64 autograd of conv2d.
 """
 import numpy as np
 import pytest
@@ -206,3 +210,91 @@ def test_conv_autograd_matmul_reference_full_batch(dev, prec):
     tolX = (2 * Cout * U32 + EXTRA[prec]) * bX
     assert ((W.grad.double() - W64.grad).abs() <= tolW).all()
     assert ((X.grad.double() - X64.grad).abs() <= tolX).all()
+
+
+class ShalomConv2d(torch.autograd.Function):
+    """Y = conv2d(X, W) (NCHW, OIHW, stride 1) on the implicit-GEMM kernels
+    (test-only): forward LibShalom_sconv2d_forward, backward
+    LibShalom_sconv2d_backward (dX and dW in one launch)."""
+
+    @staticmethod
+    def forward(ctx, X, W, pad, prec):
+        batch, C, H, Wd = X.shape
+        Cout, _, R, S = W.shape
+        P, Q = ls.conv2d_output_size(H, Wd, R, S, (1, 1), (pad, pad), (1, 1))
+        Y = torch.empty(batch, Cout, P, Q, device=X.device, dtype=torch.float32)
+        ls.sconv2d_forward(Y, W, X, batch, C, H, Wd, Cout, R, S, (1, 1), (pad, pad), (1, 1), prec)
+        ctx.save_for_backward(X, W)
+        ctx.dims = (pad, prec)
+        return Y
+
+    @staticmethod
+    def backward(ctx, dY):
+        X, W = ctx.saved_tensors
+        pad, prec = ctx.dims
+        batch, C, H, Wd = X.shape
+        Cout, _, R, S = W.shape
+        dX, dW = torch.empty_like(X), torch.empty_like(W)
+        ls.sconv2d_backward(dX, dW, dY.contiguous(), W, X, batch, C, H, Wd, Cout, R, S, (1, 1), (pad, pad),
+                            (1, 1), prec)
+        return dX, dW, None, None
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+def test_implicit_conv_autograd_finite_differences(dev, prec):
+    rng = np.random.default_rng(7)
+    batch, C, H, Wd, Cout, R = 2, 3, 6, 7, 4, 3
+    X = rng.standard_normal((batch, C, H, Wd)).astype(np.float32)
+    W = (rng.standard_normal((Cout, C, R, R)) * np.sqrt(2.0 / (C * R * R))).astype(np.float32)
+    G = rng.standard_normal((batch, Cout, H, Wd)).astype(np.float32)
+    Xt = torch.tensor(X, device=dev, requires_grad=True)
+    Wt = torch.tensor(W, device=dev, requires_grad=True)
+    (ShalomConv2d.apply(Xt, Wt, 1, prec) * torch.tensor(G, device=dev)).sum().backward()
+    gX, gW = Xt.grad.double().cpu().numpy(), Wt.grad.double().cpu().numpy()
+    _, aX, _, aW = oracle.sconv2d_bwd_ref(X, W, G, (1, 1), (1, 1), (1, 1))
+
+    def L(Xm, Wm):
+        Y, _ = oracle.sconv2d_ref(Xm, Wm, (1, 1), (1, 1), (1, 1))
+        return float(np.sum(Y * G.astype(np.float64)))
+
+    # the loss is bilinear, so the central difference is exact up to rounding;
+    # the perturbed operands are rounded to float32 by the oracle, so the
+    # step is a power of two that float32 represents exactly around each entry
+    h = 2.0 ** -10
+    for _ in range(8):
+        idx = tuple(int(rng.integers(n)) for n in X.shape)
+        Xp, Xm = X.copy(), X.copy()
+        Xp[idx] += h
+        Xm[idx] -= h
+        f = (L(Xp, W) - L(Xm, W)) / (Xp[idx] - Xm[idx])
+        tol = (2 * Cout * R * R * U32 + EXTRA[prec]) * aX[idx] + 1e-6
+        assert abs(gX[idx] - f) <= tol, ("X", idx, gX[idx], f)
+    for _ in range(8):
+        idx = tuple(int(rng.integers(n)) for n in W.shape)
+        Wp, Wm = W.copy(), W.copy()
+        Wp[idx] += h
+        Wm[idx] -= h
+        f = (L(X, Wp) - L(X, Wm)) / (Wp[idx] - Wm[idx])
+        tol = (2 * batch * H * Wd * U32 + EXTRA[prec]) * aW[idx] + 1e-6
+        assert abs(gW[idx] - f) <= tol, ("W", idx, gW[idx], f)
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+def test_implicit_conv_autograd_matches_torch_full_layer(dev, prec):
+    """The first 3 x 3 layer at batch 32: gradients against torch's float64
+    autograd of conv2d (computed on the CPU)."""
+    batch, C, HW, Cout = 32, 64, 56, 64
+    g = torch.Generator(device="cpu").manual_seed(23)
+    X = torch.randn(batch, C, HW, HW, generator=g)
+    W = torch.randn(Cout, C, 3, 3, generator=g) * (2.0 / (C * 9)) ** 0.5
+    G = torch.randn(batch, Cout, HW, HW, generator=g)
+    Xd, Wd = X.to(dev).requires_grad_(), W.to(dev).requires_grad_()
+    (ShalomConv2d.apply(Xd, Wd, 1, prec) * G.to(dev)).sum().backward()
+    X64, W64 = X.double().requires_grad_(), W.double().requires_grad_()
+    (torch.nn.functional.conv2d(X64, W64, padding=1) * G.double()).sum().backward()
+    Xa, Wa = X.double().abs().requires_grad_(), W.double().abs().requires_grad_()
+    (torch.nn.functional.conv2d(Xa, Wa, padding=1) * G.double().abs()).sum().backward()
+    tolX = (2 * Cout * 9 * U32 + EXTRA[prec]) * Xa.grad
+    tolW = (2 * batch * HW * HW * U32 + EXTRA[prec]) * Wa.grad
+    assert ((Xd.grad.double().cpu() - X64.grad).abs() <= tolX).all()
+    assert ((Wd.grad.double().cpu() - W64.grad).abs() <= tolW).all()
