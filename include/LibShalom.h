@@ -187,8 +187,8 @@ int LibShalom_sgemm_backward_strided(int transa, int transb, const float *dC, co
  * precision flags of the float GEMMs apply.  One launch; long reductions are
  * cut into slices of whole channels and summed inside it in a fixed order.
  * Asynchronous on `stream`.  LIBSHALOM_NOT_SUPPORTED when one image
- * (C*H*W floats) reaches 2 GiB, P*Q*Q or (C*R*S + R*S)*R*S reaches 2^32, or
- * H + 2 pad_h (W + 2 pad_w) reaches 2^29. */
+ * (C*H*W floats) reaches 2 GiB, P*Q*Q or (C*R*S + R*S)*R*S reaches 2^32,
+ * H + 2 pad_h (W + 2 pad_w) reaches 2^29, or C*R*S or P*Q reaches 2^21. */
 int LibShalom_sconv2d_forward(float *Y, const float *W, const float *X, long batch, long C, long H, long Wd,
                               long Cout, long R, long S, long stride_h, long stride_w, long pad_h, long pad_w,
                               long dil_h, long dil_w, int precision, LibShalom_stream_t stream);
@@ -200,7 +200,8 @@ int LibShalom_sconv2d_forward(float *Y, const float *W, const float *X, long bat
  * computed as the convolution of dY with the flipped, transposed filter,
  * written by the library into its per-stream workspace.  Split reductions
  * are summed inside the launch in a fixed order.  Limits as for the forward,
- * with H + 2 pad_h (W + 2 pad_w) below 2^28 and one dY image below 2 GiB. */
+ * with H + 2 pad_h (W + 2 pad_w) below 2^28, one dY image below 2 GiB, and
+ * Cout*R*S and H*W also below 2^21. */
 int LibShalom_sconv2d_backward(float *dX, float *dW, const float *dY, const float *W, const float *X, long batch,
                                long C, long H, long Wd, long Cout, long R, long S, long stride_h, long stride_w,
                                long pad_h, long pad_w, long dil_h, long dil_w, int precision,

@@ -1374,7 +1374,9 @@ int LibShalom_sconv2d_forward(float *Y, const float *Wt, const float *X, long ba
     const double K = (double)C * R * S, img = (double)C * H * W, PQ = (double)P * Q;
     if (batch > INT32_MAX || Cout > INT32_MAX || 4.0 * img >= 2147483648.0 || PQ >= 2147483648.0 ||
         (K + R * S) * R * S >= 4294967296.0 || PQ * Q >= 4294967296.0 || H + 2.0 * pad_h >= 536870912.0 ||
-        W + 2.0 * pad_w >= 536870912.0)
+        W + 2.0 * pad_w >= 536870912.0 ||
+        // leading dimensions of the filter (C*R*S) and of each output image (P*Q)
+        K >= (double)kMaxLd || PQ >= (double)kMaxLd)
         return set_status(LIBSHALOM_NOT_SUPPORTED);
     if (batch == 0) return set_status(LIBSHALOM_SUCCESS);
     if (!Y || !Wt || !X) return set_status(LIBSHALOM_INVALID_POINTER);
@@ -1398,7 +1400,10 @@ int LibShalom_sconv2d_backward(float *dX, float *dWt, const float *dY, const flo
         (K + R * S) * R * S >= 4294967296.0 || PQ * Q >= 4294967296.0 || H + 2.0 * pad_h >= 268435456.0 ||
         W + 2.0 * pad_w >= 268435456.0 || 4.0 * Cout * PQ >= 2147483648.0 || K >= 2147483648.0 ||
         // the input gradient's own gather: dY as the image, H x W as its output
-        ((double)Cout * R * S + R * S) * R * S >= 4294967296.0 || (double)H * W * W >= 4294967296.0)
+        ((double)Cout * R * S + R * S) * R * S >= 4294967296.0 || (double)H * W * W >= 4294967296.0 ||
+        // leading dimensions: dW and the flipped filter (C*R*S, Cout*R*S), dY and dX images (P*Q, H*W)
+        K >= (double)kMaxLd || (double)Cout * R * S >= (double)kMaxLd || PQ >= (double)kMaxLd ||
+        (double)H * W >= (double)kMaxLd)
         return set_status(LIBSHALOM_NOT_SUPPORTED);
     if (dX && (stride_h != 1 || stride_w != 1)) return set_status(LIBSHALOM_NOT_SUPPORTED);
     if (batch == 0 || (!dX && !dWt)) return set_status(LIBSHALOM_SUCCESS);

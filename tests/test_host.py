@@ -287,6 +287,8 @@ CONV_ARG_CASES = [
     ((1, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 4), 1),    # FP64 has no convolution
     ((1, 1 << 20, 32, 32, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 5),  # one image reaches 2 GiB
     ((1, 1, 1 << 28, 1, 1, 1, 1, 1, 1, 1 << 27, 0, 1, 1, 0), 5),  # H + 2 pad reaches 2^29
+    ((1, 1, 1449, 1449, 1, 1, 1, 1, 1, 0, 0, 1, 1, 0), 5),      # P*Q reaches 2^21 (output ld)
+    ((1, 233017, 1, 1, 1, 3, 3, 1, 1, 1, 1, 1, 1, 0), 5),       # C*R*S reaches 2^21 (filter ld)
     ((0, 3, 8, 8, 4, 3, 3, 1, 1, 1, 1, 1, 1, 0), 0),    # empty batch: nothing to do
 ]
 
@@ -334,3 +336,12 @@ def test_conv_backward_argument_errors_before_device():
                                         1, 1, 0, 1, 1, 1, 0, None) == 1  # P = 0
     assert L.LibShalom_sconv2d_backward(0x1000, 0x1000, 0x1000, 0x1000, 0x1000, 0, 3, 8, 8, 4, 3, 3,
                                         1, 1, 1, 1, 1, 1, 0, None) == 0  # empty batch
+
+
+def test_conv_backward_leading_dimension_limits():
+    L = ls.lib()
+    P = 0x1000
+    # Cout*R*S reaches 2^21 (the flipped filter's leading dimension)
+    assert L.LibShalom_sconv2d_backward(P, P, P, P, P, 1, 1, 4, 4, 233017, 3, 3, 1, 1, 1, 1, 1, 1, 0, None) == 5
+    # H*W reaches 2^21 (the input gradient's leading dimension)
+    assert L.LibShalom_sconv2d_backward(P, P, P, P, P, 1, 1, 1449, 1449, 1, 1, 1, 1, 1, 0, 0, 1, 1, 0, None) == 5
