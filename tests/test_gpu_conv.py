@@ -38,6 +38,11 @@ GEOMS = [
     (1, 64, 7, 7, 32, 3, 3, (1, 1), (1, 1), (1, 1)),     # K = 576
     (2, 7, 19, 23, 9, 7, 7, (2, 2), (3, 3), (1, 1)),     # 7 x 7, stride 2 (R*S = 49)
     (1, 1, 130, 3, 2, 3, 3, (1, 1), (1, 1), (1, 1)),     # a single input channel, tall image
+    (2, 9, 9, 10, 7, 4, 8, (1, 1), (2, 4), (1, 1)),      # R*S = 32: the last size with a tap mask
+    (1, 5, 8, 15, 6, 3, 11, (1, 1), (1, 5), (1, 1)),     # R*S = 33: range compares instead
+    (1, 300, 5, 5, 20, 1, 1, (1, 1), (0, 0), (1, 1)),    # 1 x 1, K = 300 cut into slices
+    (2, 4, 17, 13, 5, 3, 3, (2, 2), (2, 1), (2, 1)),     # stride and dilation together
+    (1, 6, 12, 12, 9, 4, 4, (2, 2), (1, 1), (1, 1)),     # 4 x 4, stride 2 (dW only in the backward)
 ]
 
 
