@@ -510,9 +510,13 @@ long conv_out_size(long in, long f, long stride, long pad, long dil)
 unsigned div_magic(long d) { return d == 1 ? 0u : (unsigned)((((unsigned long long)1 << 32) + d - 1) / d); }
 
 ConvGeom conv_geom(long C, long H, long W, long R, long S, long Q, long sh, long sw, long ph, long pw, long dh,
-                   long dw)
+                   long dw, long th = 1, long tw = 1)
 {
     ConvGeom g;
+    g.th = (int)th;
+    g.tw = (int)tw;
+    g.mTh = div_magic(th);
+    g.mTw = div_magic(tw);
     g.C = (int)C;
     g.H = (int)H;
     g.W = (int)W;
@@ -545,9 +549,11 @@ __global__ void k_conv_flip(const float *__restrict__ Wt, float *__restrict__ Wf
 }
 
 // Convolution backward as implicit GEMMs in one launch (either gradient may
-// be skipped with a null pointer).  dX, for stride 1 only: the convolution of
-// dY (Cout channels, P x Q) with the flipped, transposed filter, padding
-// dil*(f-1) - pad (negative padding crops), which gives back H x W.  dW: the
+// be skipped with a null pointer).  dX: the convolution of dY (Cout
+// channels, P x Q) with the flipped, transposed filter, padding
+// dil*(f-1) - pad (negative padding crops), which gives back H x W; for
+// stride > 1 it is fractionally strided (a tap reads dY row
+// (h - pad' + r*dil) / stride only when that division is exact).  dW: the
 // sum over images of dY_i times the transposed im2col matrix of X_i.
 int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *Wt, const float *X, long batch,
                  long C, long H, long W, long Cout, long R, long S, long sh, long sw, long ph, long pw, long dh,
@@ -562,7 +568,7 @@ int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *W
     BwdProblem X_, Y_;
     std::memset(&X_, 0, sizeof(X_));
     std::memset(&Y_, 0, sizeof(Y_));
-    ConvGeom gx = conv_geom(Cout, P, Q, R, S, W, 1, 1, dh * (R - 1) - ph, dw * (S - 1) - pw, dh, dw);
+    const ConvGeom gx = conv_geom(Cout, P, Q, R, S, W, 1, 1, dh * (R - 1) - ph, dw * (S - 1) - pw, dh, dw, sh, sw);
     const ConvGeom gy = conv_geom(C, H, W, R, S, Q, sh, sw, ph, pw, dh, dw);
     if (dX) {
         void *wf;
@@ -1405,7 +1411,11 @@ int LibShalom_sconv2d_backward(float *dX, float *dWt, const float *dY, const flo
         K >= (double)kMaxLd || (double)Cout * R * S >= (double)kMaxLd || PQ >= (double)kMaxLd ||
         (double)H * W >= (double)kMaxLd)
         return set_status(LIBSHALOM_NOT_SUPPORTED);
-    if (dX && (stride_h != 1 || stride_w != 1)) return set_status(LIBSHALOM_NOT_SUPPORTED);
+    // the input gradient of a strided convolution: filters of at most 32
+    // taps (its gather uses the tap mask) and strides up to 8 (exact
+    // multiply-high division of r*dil by the stride)
+    if (dX && (stride_h != 1 || stride_w != 1) && (R * S > 32 || stride_h > 8 || stride_w > 8))
+        return set_status(LIBSHALOM_NOT_SUPPORTED);
     if (batch == 0 || (!dX && !dWt)) return set_status(LIBSHALOM_SUCCESS);
     if (!dY || (dX && !Wt) || (dWt && !X)) return set_status(LIBSHALOM_INVALID_POINTER);
     return set_status(conv_bwd_run(kprec(precision), dX, dWt, dY, Wt, X, batch, C, H, W, Cout, R, S, stride_h,

@@ -62,13 +62,23 @@ struct GemmDesc {
 // map reads X[c][p*sh - ph + r*dh][q*sw - pw + s*dw] for filter tap
 // (c, r, s), or 0 where that falls in the padding.  The im2col matrix of an
 // image is (C*R*S) x (P*Q): row t = (c*R + r)*S + s, column n = p*Q + q.
+//
+// Fractionally strided form (th or tw > 1; the input gradient of a strided
+// convolution, filter taps of at most 32): tap (r, s) of output pixel (p, q)
+// reads X[c][(p*sh - ph + r*dh) / th][(q*sw - pw + s*dw) / tw] when both
+// divisions are exact, and contributes nothing otherwise.
 struct ConvGeom {
     int C, H, W;            // input channels, height, width
     int R, S, RS;           // filter height, width; R x S
     int Q;                  // output width
     int sh, sw, ph, pw, dh, dw;  // stride, padding, dilation
+    int th, tw;             // fractional strides (1: a plain convolution)
     unsigned mRS, mS, mQ;   // ceil(2^32 / d) for d = RS, S, Q; 0 when d == 1
+    unsigned mTh, mTw;      // the same for th, tw
 };
+
+// floor(x / d) for d > 0 and any sign of x.
+__device__ __forceinline__ int floordiv(int x, int d) { return x >= 0 ? x / d : -((d - 1 - x) / d); }
 
 // How a job source's B operand is formed: read from memory; gathered as a
 // convolution's im2col matrix (k = filter tap, column = output pixel); or,
@@ -391,19 +401,46 @@ struct Stage {
             const bool act = (NVEC % NT == 0) | (idx < NVEC);
             const int n = n0 + row;
             const int p = udiv(n, g.mQ), q = n - p * g.Q;
-            const int ih0 = (act & (row < N)) ? p * g.sh - g.ph : (1 << 29), iw0 = q * g.sw - g.pw;
+            const bool rok = act & (row < N);
+            int ih0, iw0;
+            unsigned taps = 0;
+            if (g.th == 1 && g.tw == 1) {
+                ih0 = rok ? p * g.sh - g.ph : (1 << 29);
+                iw0 = q * g.sw - g.pw;
+                if (g.RS <= 32) {
+                    unsigned cols = 0;
+                    for (int s = 0; s < g.S; ++s)
+                        cols |= (unsigned)((unsigned)(iw0 + s * g.dw) < (unsigned)g.W) << s;
+                    for (int r = 0; r < g.R; ++r)
+                        if ((unsigned)(ih0 + r * g.dh) < (unsigned)g.H) taps |= cols << (r * g.S);
+                }
+            } else {
+                // u = p*sh - ph = a*th + al (floor division); a tap with
+                // r*dh = b*th + be contributes when al + be is 0 or th, and
+                // then reads input row a + (al > 0) + b: the row splits into
+                // a per-pixel part (kept in ih0) and a per-tap part b.
+                const int u = p * g.sh - g.ph, v = q * g.sw - g.pw;
+                const int a = floordiv(u, g.th), al = u - a * g.th;
+                const int a2 = floordiv(v, g.tw), al2 = v - a2 * g.tw;
+                ih0 = rok ? a + (al > 0) : (1 << 29);
+                iw0 = a2 + (al2 > 0);
+                unsigned cols = 0;
+                for (int s = 0; s < g.S; ++s) {
+                    const int b = s * g.dw / g.tw, be = s * g.dw - b * g.tw;
+                    const bool hit = (al2 + be == 0) | (al2 + be == g.tw);
+                    cols |= (unsigned)(hit & ((unsigned)(iw0 + b) < (unsigned)g.W)) << s;
+                }
+                for (int r = 0; r < g.R; ++r) {
+                    const int b = r * g.dh / g.th, be = r * g.dh - b * g.th;
+                    const bool hit = (al + be == 0) | (al + be == g.th);
+                    if (hit & ((unsigned)(ih0 + b) < (unsigned)g.H)) taps |= cols << (r * g.S);
+                }
+            }
             // modulo 2^32: only used where ih0 is a real row; kept opaque so
             // that the compiler adds a tap's offset to it instead of
             // re-deriving (ih0 + dy) * W with a quarter-rate multiply
             int base = (int)(((unsigned)ih0 * (unsigned)g.W + (unsigned)iw0) * ES);
             asm volatile("" : "+v"(base));
-            unsigned taps = 0;
-            if (g.RS <= 32) {
-                unsigned cols = 0;
-                for (int s = 0; s < g.S; ++s) cols |= (unsigned)((unsigned)(iw0 + s * g.dw) < (unsigned)g.W) << s;
-                for (int r = 0; r < g.R; ++r)
-                    if ((unsigned)(ih0 + r * g.dh) < (unsigned)g.H) taps |= cols << (r * g.S);
-            }
             px.ih0[i] = ih0;
             px.iw0[i] = iw0;
             px.base[i] = base;
@@ -439,7 +476,10 @@ struct Stage {
         const int tl = k0 + (tid & (BK - 1));
         const int tc = udiv(tl, g.mRS), trem = tl - tc * g.RS;
         const int tr = udiv(trem, g.mS), ts = trem - tr * g.S;
-        const int ltoff = (tc * HW + tr * g.dh * g.W + ts * g.dw) * ES;
+        // the tap's input row / column offsets (fractionally strided: the
+        // per-tap parts r*dh / th, s*dw / tw)
+        const int tdy = udiv(tr * g.dh, g.mTh), tdx = udiv(ts * g.dw, g.mTw);
+        const int ltoff = (tc * HW + tdy * g.W + tdx) * ES;
         int off[PER][VEC];
         if (g.RS <= 32) {
             const unsigned lbit = tl < kend ? 1u << trem : 0u;  // no bit for a tap past K

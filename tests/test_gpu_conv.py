@@ -42,7 +42,9 @@ GEOMS = [
     (1, 5, 8, 15, 6, 3, 11, (1, 1), (1, 5), (1, 1)),     # R*S = 33: range compares instead
     (1, 300, 5, 5, 20, 1, 1, (1, 1), (0, 0), (1, 1)),    # 1 x 1, K = 300 cut into slices
     (2, 4, 17, 13, 5, 3, 3, (2, 2), (2, 1), (2, 1)),     # stride and dilation together
-    (1, 6, 12, 12, 9, 4, 4, (2, 2), (1, 1), (1, 1)),     # 4 x 4, stride 2 (dW only in the backward)
+    (1, 6, 12, 12, 9, 4, 4, (2, 2), (1, 1), (1, 1)),     # 4 x 4, stride 2
+    (1, 3, 11, 10, 4, 3, 3, (3, 3), (1, 0), (1, 1)),     # stride 3
+    (1, 3, 16, 16, 4, 5, 5, (4, 2), (2, 2), (1, 1)),     # stride (4, 2), 5 x 5
 ]
 
 
@@ -146,9 +148,14 @@ def test_conv_empty_batch_is_a_no_op():
     assert (y.cpu() == 3.0).all()
 
 
-# Backward: stride-1 geometries (dX and dW) and strided ones (dW only).
+# Backward: stride-1 geometries, and strided ones (dX through the fractionally
+# strided gather for filters of at most 32 taps; dW alone beyond that).
 BWD_GEOMS = [g for g in GEOMS if g[7] == (1, 1)]
 BWD_STRIDED = [g for g in GEOMS if g[7] != (1, 1)]
+
+
+def _strided_dx_supported(g):
+    return g[5] * g[6] <= 32
 
 
 def _run_bwd(X, Wt, dY, st, pad, dil, prec, want_dx=True, guard=64):
@@ -185,11 +192,11 @@ def test_conv_backward_matches_oracle(g, prec):
     _bwd_case(g, prec, True)
 
 
-@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
+@pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("g", BWD_STRIDED)
-def test_conv_backward_strided_filter_gradient(g, prec):
-    print(f"conv backward (dW only) {g} precision {prec}", flush=True)
-    _bwd_case(g, prec, False)
+def test_conv_backward_strided(g, prec):
+    print(f"conv backward (strided) {g} precision {prec}", flush=True)
+    _bwd_case(g, prec, _strided_dx_supported(g))
 
 
 def _torch_grads(X, Wt, dY):
