@@ -390,12 +390,19 @@ void bwd_slice(BwdProblem &P, int T, long len)
 
 // Tile size and reduction slices for problems that share one launch.
 // 128 x 128 tiles when every output dimension fills them and they alone give
-// every compute unit a unit of work, else 64 x 64.  A reduction is cut into
+// at least big_quarters / 4 of the compute units a unit of work (reduction
+// slices fill the rest), else 64 x 64.  (Measured on the conv layers with
+// three quarters: the 128x784 layer, 224 big tiles for 256 CUs, runs
+// 1.1-1.35x faster on them; the 256x196 layer forward, 128 big tiles, runs
+// slower, FP32 by 1.5x -- but its implicit backward, whose dW problem adds
+// tiles, 1.2x faster in BF16X3 (and 7% slower in FP32), so that launch plans
+// with one half for the split-BF16 modes.)  A reduction is cut into
 // slices of about W / (2 * CUs) (W = all tiles' reduction work) so that no
 // unit is much longer than a compute unit's share; slices cost partial-tile
 // traffic, which is kept under a quarter of the operand traffic.  Returns
 // the tile size (0: nothing to launch, < 0: -status).
-int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu, hipStream_t s)
+int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu, hipStream_t s,
+                  int big_quarters = 3)
 {
     auto base_units = [&](int T) {
         long u = 0;
@@ -405,7 +412,7 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
         }
         return u;
     };
-    const int T = (allow_big && base_units(128) >= num_cu) ? 128 : 64;
+    const int T = (allow_big && 4 * base_units(128) >= (long)big_quarters * num_cu) ? 128 : 64;
     double W = 0, ops = 0;
     for (int i = 0; i < n; ++i) {
         const BwdProblem &P = *probs[i];
@@ -588,7 +595,7 @@ int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *W
     if (dW) probs[n++] = &Y_;
     long mind = LONG_MAX;
     for (int i = 0; i < n; ++i) mind = std::min(mind, (long)std::min(probs[i]->R, probs[i]->Cc));
-    const int T = plan_problems(probs, n, mind >= 128, dev, info.num_cu, s);
+    const int T = plan_problems(probs, n, mind >= 128, dev, info.num_cu, s, prec == P_FP32 ? 3 : 2);
     if (T <= 0) return -T;
     ConvBwdLaunch fn = bwd_table().conv_bwd[prec][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
