@@ -234,7 +234,9 @@ int LibShalom_query_tuned_tile(int precision, int transa, int transb, long M, lo
 /* Planner introspection (host only, no device needed): the workgroup tile
  * (tile_m x tile_n of C), K-split count and K slice length the library would
  * use for one GEMM on a device with num_cu compute units (mp != 0: the
- * LibShalom_sgemm_mp path), and the tile a batched plan gives one GEMM.
+ * LibShalom_sgemm_mp path), and the tile a batched plan gives one GEMM
+ * (FP64 plans run the GEMMs of the 32 x 32 bucket with M, N <= 16 on
+ * one-wave 16 x 16 tiles; both tile queries report that).
  * Replaces the reference's thread-grid planner Dete_grad_N_threads_nums /
  * Dete_grad_M_threads_nums (SMM_thread.c:14-72). */
 int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_cu, int *tile_m,

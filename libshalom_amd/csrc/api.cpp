@@ -766,6 +766,8 @@ void order_jobs(std::vector<GemmDesc> &jobs, int grid, int bk)
     for (size_t r0 = G; r0 + G <= jobs.size(); r0 += 2 * G) std::reverse(jobs.begin() + r0, jobs.begin() + r0 + G);
 }
 
+constexpr int kTile16 = -16;  // bucket tile id of the FP64 16 x 16 persistent kernel
+
 struct Bucket {
     int tile;
     std::vector<GemmDesc> jobs;
@@ -1068,6 +1070,16 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
         for (int m0 = 0; m0 < d.M; m0 += ts.bm)
             for (int n0 = 0; n0 < d.N; n0 += ts.bn) b.jobs.push_back(tile_job(ta, tb, d, m0, n0, ts.bm, ts.bn, esize));
     }
+    // FP64: the GEMMs of the 32 x 32 bucket with M, N <= 16 go to one-wave
+    // 16 x 16 tiles (bucket index NTILE, tile id kTile16).
+    buckets.emplace_back();
+    buckets[NTILE].tile = kTile16;
+    if (f64 && mode != BM_SINGLE && launch_table().persist16[ta][tb]) {
+        std::vector<GemmDesc> keep;
+        for (const GemmDesc &j : buckets[T32x32].jobs)
+            (j.M <= 16 && j.N <= 16 ? buckets[NTILE].jobs : keep).push_back(j);
+        buckets[T32x32].jobs.swap(keep);
+    }
     // Per bucket: as many workgroups as can be resident, jobs dealt so that
     // their totals balance.
     long total_jobs = 0;
@@ -1077,7 +1089,8 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
         if (nj == 0) continue;
         if (nj > (long)INT32_MAX / 2) return LIBSHALOM_NOT_SUPPORTED;
         total_jobs += nj;
-        PersistOccupancy occ = launch_table().persist_occ[prec][ta][tb][b.tile];
+        PersistOccupancy occ = b.tile == kTile16 ? launch_table().persist16_occ[ta][tb]
+                                                 : launch_table().persist_occ[prec][ta][tb][b.tile];
         const long slots = (long)std::max(1, occ ? occ() : 1) * info.num_cu;
         b.grid = (int)std::min(nj, slots);
         order_jobs(b.jobs, b.grid, f64 ? 16 : 32);
@@ -1117,19 +1130,20 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     // start on compute units the large one's last workgroups leave idle
     // (measured: 2.8% less time per step for the 50k BF16X3 batch, 4% for
     // BF16X6).  Streams keep the default priority.
-    const int order[NTILE] = {T128x128, T128x64, T128x32, T32x128, T64x64, T32x32};
-    hipStream_t fs[NTILE] = {};
-    hipEvent_t ev[NTILE + 1] = {};
+    constexpr int NB = NTILE + 1;  // the tile buckets and the FP64 16 x 16 one
+    const int order[NB] = {T128x128, T128x64, T128x32, T32x128, T64x64, T32x32, NTILE};
+    hipStream_t fs[NB] = {};
+    hipEvent_t ev[NB + 1] = {};
     auto release = [&]() {
-        for (int i = 0; i < NTILE; ++i)
+        for (int i = 0; i < NB; ++i)
             if (fs[i]) (void)hipStreamDestroy(fs[i]);
-        for (int i = 0; i <= NTILE; ++i)
+        for (int i = 0; i <= NB; ++i)
             if (ev[i]) (void)hipEventDestroy(ev[i]);
     };
     bool launch_ok = hipStreamCreateWithFlags(&cs, hipStreamNonBlocking) == hipSuccess &&
-                     hipEventCreateWithFlags(&ev[NTILE], hipEventDisableTiming) == hipSuccess;
+                     hipEventCreateWithFlags(&ev[NB], hipEventDisableTiming) == hipSuccess;
     int lead = -1;  // the bucket on the capture stream
-    for (int oi = 0; oi < NTILE && launch_ok; ++oi) {
+    for (int oi = 0; oi < NB && launch_ok; ++oi) {
         if (buckets[order[oi]].grid == 0) continue;
         if (lead < 0) {
             lead = oi;
@@ -1144,15 +1158,16 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     }
     // the fork point (only when there is a bucket to fork)
     bool forked = false;
-    for (int oi = 0; oi < NTILE; ++oi) forked |= fs[oi] != nullptr;
-    if (forked) launch_ok = hipEventRecord(ev[NTILE], cs) == hipSuccess;
-    for (int oi = 0; oi < NTILE && launch_ok; ++oi) {
+    for (int oi = 0; oi < NB; ++oi) forked |= fs[oi] != nullptr;
+    if (forked) launch_ok = hipEventRecord(ev[NB], cs) == hipSuccess;
+    for (int oi = 0; oi < NB && launch_ok; ++oi) {
         const Bucket &b = buckets[order[oi]];
         if (b.grid == 0) continue;
         const GemmDesc *jobs = (const GemmDesc *)((char *)plan->dev_buf + b.off);
-        PersistLaunch fn = launch_table().persist[prec][ta][tb][b.tile];
+        PersistLaunch fn = b.tile == kTile16 ? launch_table().persist16[ta][tb]
+                                             : launch_table().persist[prec][ta][tb][b.tile];
         const hipStream_t bs = oi == lead ? cs : fs[oi];
-        if (oi != lead) launch_ok = hipStreamWaitEvent(bs, ev[NTILE], 0) == hipSuccess;
+        if (oi != lead) launch_ok = hipStreamWaitEvent(bs, ev[NB], 0) == hipSuccess;
         if (!launch_ok || !fn || fn(jobs, (int)b.jobs.size(), b.grid, bs) != hipSuccess) launch_ok = false;
         if (launch_ok && oi != lead)
             launch_ok = hipEventRecord(ev[oi], bs) == hipSuccess && hipStreamWaitEvent(cs, ev[oi], 0) == hipSuccess;
@@ -1493,6 +1508,15 @@ int LibShalom_autotune_reset(void)
     return set_status(LIBSHALOM_SUCCESS);
 }
 
+// The tile a batched plan runs a GEMM on, given its bucket tile: FP64 plans
+// run the GEMMs of the 32 x 32 bucket with M, N <= 16 on 16 x 16 tiles.
+static void report_plan_tile(int precision, int t, long M, long N, int *tile_m, int *tile_n)
+{
+    const bool t16 = precision == LIBSHALOM_FP64 && t == T32x32 && M <= 16 && N <= 16;
+    if (tile_m) *tile_m = t16 ? 16 : kTileShapes[t].bm;
+    if (tile_n) *tile_n = t16 ? 16 : kTileShapes[t].bn;
+}
+
 int LibShalom_query_tuned_tile(int precision, int transa, int transb, long M, long N, long K, int *tile_m,
                                int *tile_n)
 {
@@ -1500,8 +1524,7 @@ int LibShalom_query_tuned_tile(int precision, int transa, int transb, long M, lo
     if ((transa != 0 && transa != 1) || (transb != 0 && transb != 1) || M < 0 || N < 0 || K < 0)
         return set_status(LIBSHALOM_INVALID_VALUE);
     const int t = tuned_tile(kprec(precision), transa, transb, M, N, K);
-    if (tile_m) *tile_m = kTileShapes[t].bm;
-    if (tile_n) *tile_n = kTileShapes[t].bn;
+    report_plan_tile(precision, t, M, N, tile_m, tile_n);
     return set_status(LIBSHALOM_SUCCESS);
 }
 
@@ -1523,8 +1546,7 @@ int LibShalom_query_batched_tile(int precision, long M, long N, long K, int *til
     if (!valid_precision(precision) && precision != LIBSHALOM_FP64) return set_status(LIBSHALOM_INVALID_VALUE);
     if (M < 0 || N < 0 || K < 0) return set_status(LIBSHALOM_INVALID_VALUE);
     const int t = batched_tile(kprec(precision), M, N, K);
-    if (tile_m) *tile_m = kTileShapes[t].bm;
-    if (tile_n) *tile_n = kTileShapes[t].bn;
+    report_plan_tile(precision, t, M, N, tile_m, tile_n);
     return set_status(LIBSHALOM_SUCCESS);
 }
 

@@ -279,9 +279,11 @@ struct TileCfg {
     // (the rows split) for tall tiles and one row for wide ones, so the tall-
     // and short-skinny GEMMs those tiles serve keep every wave on valid rows
     // or columns.
-    static constexpr int NT = (BM == 128 && BN == 128) ? 512 : 256;
+    // (The 16 x 16 tile, used for the smallest FP64 GEMMs of a batch, is one
+    // wave.)
+    static constexpr int NT = (BM == 128 && BN == 128) ? 512 : (BM == 16 && BN == 16 ? 64 : 256);
     static constexpr int NW = NT / 64;
-    static constexpr int GN = BM == BN ? 2 : (BM > BN ? 1 : NW), GM = NW / GN;
+    static constexpr int GN = BM == BN ? (NW >= 2 ? 2 : 1) : (BM > BN ? 1 : NW), GM = NW / GN;
     static constexpr int WM = BM / 16 / GM, WN = BN / 16 / GN;
     // Waves per SIMD the persistent kernel is compiled for: four for the
     // eight-wave tiles of FP32 / BF16 / BF16X3 (two workgroups per CU, at
@@ -291,7 +293,7 @@ struct TileCfg {
     static constexpr int WAVES_PER_EU = (NT == 512 && PREC != P_BF16X6 && PREC != P_FP64) ? 4
                                         : (BM * BN >= 128 * 128 ? 2 : 1);
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
-    static_assert(BM % 32 == 0 && BN % 32 == 0, "tile must be a multiple of 32");
+    static_assert(BM % 16 == 0 && BN % 16 == 0, "tile must be a multiple of 16");
     static_assert(WM >= 1 && WN >= 1 && WM * GM * 16 == BM && WN * GN * 16 == BN, "the wave grid covers the tile");
     static_assert(WM * WN <= 16, "at most 16 sub-tiles per wave");
     static_assert(A_BYTES % 16 == 0 && B_BYTES % 16 == 0, "panels stay 16-byte aligned");

@@ -89,6 +89,10 @@ struct LaunchTable {
     TilesLaunch tiles[NPREC][2][2][NTILE];
     PersistLaunch persist[NPREC][2][2][NTILE];
     PersistOccupancy persist_occ[NPREC][2][2][NTILE];
+    // FP64 batched plans only: one-wave 16 x 16 tiles for the GEMMs with
+    // M, N <= 16 (a 32 x 32 tile's four waves would leave three idle)
+    PersistLaunch persist16[2][2];
+    PersistOccupancy persist16_occ[2][2];
 };
 LaunchTable &launch_table();
 
@@ -144,6 +148,11 @@ void register_trans(LaunchTable &t)
     register_one<PREC, TA, TB, 128, 32>(t, T128x32);
     register_one<PREC, TA, TB, 128, 64>(t, T128x64);
     register_one<PREC, TA, TB, 32, 128>(t, T32x128);
+    if constexpr (PREC == P_FP64) {
+        typedef TileCfg<PREC, TA, TB, 16, 16> C16;
+        t.persist16[TA][TB] = &launch_persistent<C16>;
+        t.persist16_occ[TA][TB] = &persistent_occupancy<C16>;
+    }
 }
 
 // Defined in one translation unit per precision and transa.

@@ -237,6 +237,8 @@ def test_planner_errors():
 @pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3, ls.FP64])
 def test_batched_tile_rule(prec):
     assert ls.query_batched_tile(prec, 5, 32, 100) == (32, 32)
+    # FP64: one-wave 16 x 16 tiles for GEMMs with M, N <= 16
+    assert ls.query_batched_tile(prec, 13, 16, 100) == ((16, 16) if prec == ls.FP64 else (32, 32))
     assert ls.query_batched_tile(prec, 33, 10, 1) == (64, 64)
     big = ls.query_batched_tile(prec, 100, 128, 7)
     assert big == ((64, 64) if prec == ls.FP64 else (128, 128))
