@@ -371,3 +371,47 @@ def test_batched_large_tiles_all_trans(dev, prec, ta, tb):
     bad, st = oracle.check_kv(outs[0][:-1], Cref, S, Ke, U32, PREC_EXTRA[prec])
     assert bad == 0, st
     assert np.isnan(outs[0][-1])
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_plan_replays_with_eager_work_between(dev, f64):
+    """A plan's graph runs its buckets on parallel branches.  Replay it four
+    times with unrelated eager work on the same stream in between (a single
+    GEMM and a fill of C), and check every replay against the oracle and bit
+    for bit against the first."""
+    rng = np.random.default_rng(31 + int(f64))
+    if f64:
+        dims = [4, 5, 9, 13, 16, 22, 23, 26, 32]  # 16x16 and 32x32 buckets
+        shapes = [tuple(int(rng.choice(dims)) for _ in range(3)) for _ in range(400)]
+    else:
+        shapes = [tuple(int(x) for x in rng.integers(1, 129, size=3)) for _ in range(400)]  # every bucket
+    prec = ls.FP32
+    A, B, offA, offB, offC = _flat_batch(shapes, 0, 0, 17, f64)
+    dt = torch.float64 if f64 else torch.float32
+    dA, dB = dev_tensor(A, dev), dev_tensor(B, dev)
+    dC = torch.full((int(offC[-1]),), float("nan"), dtype=dt, device=dev)
+    es = dA.element_size()
+    Cs = [dC.data_ptr() + es * int(o) for o in offC[:-1]]
+    As = [dA.data_ptr() + es * int(o) for o in offA[:-1]]
+    Bs = [dB.data_ptr() + es * int(o) for o in offB[:-1]]
+    Ms, Ns, Ks = zip(*shapes)
+    Cref, S, Ke = oracle.gemm_ref_flat(0, 0, A, B, offA[:-1], offB[:-1], offC[:-1], Ms, Ns, Ks, int(offC[-1]))
+    X = torch.randn(300, 70, device=dev)
+    Y = torch.randn(70, 90, device=dev)
+    Z = torch.empty(300, 90, device=dev)
+    first = None
+    with ls.BatchedPlan(0, 0, Cs, As, Bs, Ms, Ns, Ks, precision=prec, fp64=f64) as plan:
+        for rep in range(4):
+            ls.sgemm_ex(0, 0, Z, X, Y, 300, 90, 70, ls.FP32)  # eager work on the stream
+            dC.fill_(float("nan"))
+            plan.execute()
+            if rep % 2:
+                torch.cuda.synchronize()
+            out = dC.cpu().numpy().copy()
+            bad, st = oracle.check_kv(out, Cref, S, Ke, 2.0 ** -53 if f64 else U32)
+            assert bad == 0, (rep, st)
+            if first is None:
+                first = out
+            assert np.array_equal(out, first), rep
+    ref = X.double() @ Y.double()
+    assert ((Z.double() - ref).abs() <= 2 * 70 * U32 * (X.double().abs() @ Y.double().abs())).all()
