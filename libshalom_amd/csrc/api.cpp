@@ -540,7 +540,32 @@ ConvGeom conv_geom(long C, long H, long W, long R, long S, long Q, long sh, long
     g.mRS = div_magic(R * S);
     g.mS = div_magic(S);
     g.mQ = div_magic(Q);
+    g.fold = 0;
+    g.PQ = 0;
+    g.img = 0;
+    g.mPQ = 0;
     return g;
+}
+
+// Fold the batch into the pixel dimension of a convolution GEMM when the
+// per-image pixel count leaves part of the last 64-wide tile empty and the
+// folded problem keeps 32-bit offsets and exact divisions: the input batch
+// (in_img elements per image) and the output (out_img per image) below 2 GiB,
+// and batch * pixels * pixels below 2^32.
+bool conv_fold(long batch, long pixels, long in_img, long out_img)
+{
+    return batch > 1 && pixels % 64 != 0 && 4.0 * batch * in_img < 2147483648.0 &&
+           4.0 * batch * out_img < 2147483648.0 && (double)batch * pixels * pixels < 4294967296.0;
+}
+
+void fold_into(BwdProblem &P, ConvGeom &g, long batch, long pixels, long in_img, long out_img)
+{
+    g.fold = (int)batch;
+    g.PQ = (int)pixels;
+    g.img = (int)in_img;
+    g.mPQ = div_magic(pixels);
+    P.fold_total = (int)(batch * out_img);
+    P.fold_mpq = g.mPQ;
 }
 
 // Wf[c][o][R-1-r][S-1-s] = W[o][c][r][s]: the filter of the input gradient
@@ -575,7 +600,7 @@ int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *W
     BwdProblem X_, Y_;
     std::memset(&X_, 0, sizeof(X_));
     std::memset(&Y_, 0, sizeof(Y_));
-    const ConvGeom gx = conv_geom(Cout, P, Q, R, S, W, 1, 1, dh * (R - 1) - ph, dw * (S - 1) - pw, dh, dw, sh, sw);
+    ConvGeom gx = conv_geom(Cout, P, Q, R, S, W, 1, 1, dh * (R - 1) - ph, dw * (S - 1) - pw, dh, dw, sh, sw);
     const ConvGeom gy = conv_geom(C, H, W, R, S, Q, sh, sw, ph, pw, dh, dw);
     if (dX) {
         void *wf;
@@ -585,7 +610,15 @@ int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *W
         hipLaunchKernelGGL(k_conv_flip, dim3((unsigned)std::min(1024L, ceil_div(n, 256))), dim3(256), 0, s, Wt,
                            (float *)wf, (int)Cout, (int)C, (int)RS);
         if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
-        bwd_side(X_, wf, 0, (int)(Cout * RS), dY, Cout * PQ, 0, dX, C * HW, (int)HW, C, HW, Cout * RS, (int)batch);
+        // the input gradient's pixels are the images' H x W, batch-folded when
+        // the last tile of an image would be partly empty (on 3x3 layers,
+        // batch 32: 14x14 pixels 8-21% faster, 7x7 BF16X3 18%, others within 2%)
+        if (conv_fold(batch, HW, Cout * PQ, C * HW)) {
+            bwd_side(X_, wf, 0, (int)(Cout * RS), dY, 0, 0, dX, 0, (int)HW, C, batch * HW, Cout * RS, 1);
+            fold_into(X_, gx, batch, HW, Cout * PQ, C * HW);
+        } else {
+            bwd_side(X_, wf, 0, (int)(Cout * RS), dY, Cout * PQ, 0, dX, C * HW, (int)HW, C, HW, Cout * RS, (int)batch);
+        }
         X_.kalign = kalign;
     }
     if (dW) bwd_side(Y_, dY, Cout * PQ, (int)PQ, X, C * HW, 0, dW, 0, (int)K, Cout, K, PQ, (int)batch);
@@ -614,12 +647,14 @@ int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch
     int st = current_device(&dev, &info);
     if (st) return st;
     const long RS = R * S, K = C * RS, PQ = P * Q;
-    const ConvGeom g = conv_geom(C, H, W, R, S, Q, sh, sw, ph, pw, dh, dw);
+    ConvGeom g = conv_geom(C, H, W, R, S, Q, sh, sw, ph, pw, dh, dw);
     BwdProblem Pb;
+    // Not batch-folded: on 3x3 layers of 7x7 to 28x28 pixels, batch 32, the
+    // folded forward measured within 2% of this one either way.
     bwd_side(Pb, Wt, 0, (int)K, X, C * H * W, 0, Y, Cout * PQ, (int)PQ, Cout, PQ, K, (int)batch);
     Pb.kalign = (int)(32 / std::gcd(32L, RS) * RS);
     BwdProblem *probs[1] = {&Pb};
-    const int T = plan_problems(probs, 1, std::min(Cout, PQ) >= 128, dev, info.num_cu, s);
+    const int T = plan_problems(probs, 1, std::min(Cout, (long)Pb.Cc) >= 128, dev, info.num_cu, s);
     if (T <= 0) return -T;
     ConvLaunch fn = bwd_table().conv_fwd[prec][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;

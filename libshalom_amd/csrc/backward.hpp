@@ -39,6 +39,11 @@ struct BwdProblem {
     int units;            // workgroups of this problem
     float *slots;         // nslices > 1: partial tiles, [tile][slice][BM * BN]
     int *counters;        // nslices > 1: arrivals per tile (zero between launches)
+    // A convolution output with the batch folded into its columns (fold_total
+    // > 0): column n of the R x Cc output is pixel n % ldo of image n / ldo,
+    // at n + (n / ldo) * (R - 1) * ldo; fold_total = all output elements.
+    int fold_total;
+    unsigned fold_mpq;    // ceil(2^32 / ldo)
 };
 
 // The segments of one unit's slice as a job stream (see Tile::stream).  The
@@ -64,8 +69,16 @@ struct SliceSrc {
     void *out;        // where the tile goes: the output, or the unit's slot
     int ldo;
     ConvGeom geom;
+    FoldedCols fcols;   // IM2COL_ == kIm2col: where the tile's columns land
 
     __device__ __forceinline__ int count() const { return nseg; }
+    __device__ __forceinline__ auto cols() const
+    {
+        if constexpr (IM2COL == kIm2col)
+            return fcols;
+        else
+            return PlainCols{};
+    }
     __device__ __forceinline__ GemmDesc get(int j) const
     {
         typedef typename Cfg::T T;
@@ -108,10 +121,14 @@ __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_o
         if (c >= tcn) continue;
         f32x4 acc = *(const f32x4 *)(base + e);
         for (int s = 1; s < P.nslices; ++s) acc += *(const f32x4 *)(base + (size_t)s * (BM * BN) + e);
-        float *o = O + (long long)(r0 + r) * P.ldo + c0 + c;
+        float *o = O + (long long)(r0 + r) * P.ldo;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (c + q < tcn) o[q] = acc[q];
+        for (int q = 0; q < 4; ++q) {
+            if (c + q >= tcn) continue;
+            const int n = c0 + c + q;  // column of the output
+            const long long col = P.fold_total > 0 ? n + (long long)udiv(n, P.fold_mpq) * (P.R - 1) * P.ldo : n;
+            o[col] = acc[q];
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) P.counters[tg] = 0;  // ready for the next launch
@@ -161,8 +178,17 @@ __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, 
         src.boff = 0;
     else
         src.boff = Cfg::TB ? (long long)src.c0 * P.ldb + src.rho0 : (long long)src.rho0 * P.ldb + src.c0;
+    // the tile's columns: folded convolution output written directly, or
+    // the tile's own columns (FoldedCols with no skip and the plain extent)
+    const bool folded = IM2COL == kIm2col && P.fold_total > 0 && P.nslices == 1;
+    src.fcols.n0 = folded ? src.c0 : 0;
+    src.fcols.skip = folded ? (P.R - 1) * P.ldo : 0;
+    src.fcols.ext = folded ? P.fold_total - src.r0 * P.ldo
+                           : (min(Cfg::BM, P.R - src.r0) - 1) * (P.nslices == 1 ? P.ldo : Cfg::BN) +
+                                 min(Cfg::BN, P.Cc - src.c0);
+    src.fcols.mPQ = P.fold_mpq;
     if (P.nslices == 1) {
-        src.out = (float *)P.O + (long long)img_out * P.sO + (long long)src.r0 * P.ldo + src.c0;
+        src.out = (float *)P.O + (long long)img_out * P.sO + (long long)src.r0 * P.ldo + (folded ? 0 : src.c0);
         src.ldo = P.ldo;
     } else {
         src.out = P.slots + ((size_t)tg * P.nslices + s) * (Cfg::BM * Cfg::BN);

@@ -75,6 +75,11 @@ struct ConvGeom {
     int th, tw;             // fractional strides (1: a plain convolution)
     unsigned mRS, mS, mQ;   // ceil(2^32 / d) for d = RS, S, Q; 0 when d == 1
     unsigned mTh, mTw;      // the same for th, tw
+    // Batch folded into the pixel dimension (fold = the number of images, 0
+    // when not folded): output pixel n of the GEMM is pixel n % PQ of image
+    // n / PQ, whose input starts img elements after the previous image's.
+    int fold, PQ, img;
+    unsigned mPQ;
 };
 
 // floor(x / d) for d > 0 and any sign of x.
@@ -92,6 +97,33 @@ enum BSource : int { kPlainB = 0, kIm2col = 1, kIm2colT = 2 };
 // Written as a select rather than a branch so that scalar uses stay
 // straight-line code.
 __device__ __forceinline__ int udiv(int x, unsigned m) { return (int)__umulhi((unsigned)x, m) + (m == 0u ? x : 0); }
+
+// Where column col of a tile's C lands, relative to the tile's row origin.
+// PlainCols: the tile's C pointer already includes its first column.
+// Rows past M fall past the buffer's extent ((M - 1) * ldo + N elements).
+struct PlainCols {
+    static constexpr bool MASK_ROWS = false;
+    __device__ __forceinline__ int operator()(int col) const { return col; }
+    __device__ __forceinline__ int extent(int M, int ldo, int N) const { return (M - 1) * ldo + N; }
+};
+
+// The output of a convolution with the batch folded into its pixels
+// (NCHW, Cout rows of PQ pixels per image): column n0 + col is pixel
+// n % PQ of image n / PQ, at n + (n / PQ) * skip with skip = (Cout - 1) * PQ;
+// ext is the number of elements from the row origin to the output's end,
+// so the stores mask rows past M themselves (they would land on other
+// images' pixels).
+struct FoldedCols {
+    static constexpr bool MASK_ROWS = true;
+    int n0, skip, ext;
+    unsigned mPQ;
+    __device__ __forceinline__ int operator()(int col) const
+    {
+        const int n = n0 + col;
+        return n + udiv(n, mPQ) * skip;
+    }
+    __device__ __forceinline__ int extent(int, int, int) const { return ext; }
+};
 
 // GemmDesc::flags bit: the job is one segment of a longer reduction.  Its C
 // tile is not stored and the next job of the same stream (which must have the
@@ -401,7 +433,12 @@ struct Stage {
             int row, k;
             coords(idx, row, k);
             const bool act = (NVEC % NT == 0) | (idx < NVEC);
-            const int n = n0 + row;
+            int n = n0 + row, img_off = 0;
+            if (g.fold) {  // image n / PQ of the batch
+                const int b = udiv(n, g.mPQ);
+                n -= b * g.PQ;
+                img_off = b * g.img;
+            }
             const int p = udiv(n, g.mQ), q = n - p * g.Q;
             const bool rok = act & (row < N);
             int ih0, iw0;
@@ -441,7 +478,7 @@ struct Stage {
             // modulo 2^32: only used where ih0 is a real row; kept opaque so
             // that the compiler adds a tap's offset to it instead of
             // re-deriving (ih0 + dy) * W with a quarter-rate multiply
-            int base = (int)(((unsigned)ih0 * (unsigned)g.W + (unsigned)iw0) * ES);
+            int base = (int)(((unsigned)img_off + (unsigned)ih0 * (unsigned)g.W + (unsigned)iw0) * ES);
             asm volatile("" : "+v"(base));
             px.ih0[i] = ih0;
             px.iw0[i] = iw0;
@@ -474,7 +511,9 @@ struct Stage {
         kcut = kc;
         const int HW = g.H * g.W;
         const int nch = udiv(kend + g.RS - 1, g.mRS);  // channels the job spans
-        const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0) ? nch * HW * ES : 0);
+        // (folded: the job's channels of every image up to the last)
+        const int span = g.fold ? (g.fold - 1) * g.img + nch * HW : nch * HW;
+        const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0) ? span * ES : 0);
         const int tl = k0 + (tid & (BK - 1));
         const int tc = udiv(tl, g.mRS), trem = tl - tc * g.RS;
         const int tr = udiv(trem, g.mS), ts = trem - tr * g.S;
@@ -911,7 +950,7 @@ struct Tile {
         // job end.
         if constexpr (Cfg::NBUF == 1) __syncthreads();
         chunk_to_lds(a1, b1, lds_buf(lds, bi + 1), tid);
-        if (cur.c + 1 == cur.nch && !chained) store((T *)dc.C, dc.ldc, dc.M, dc.N);
+        if (cur.c + 1 == cur.nch && !chained) store((T *)dc.C, dc.ldc, dc.M, dc.N, src.cols());
         if (n1.job < 0) return false;
         // n1 starting a job means cur ends one: a chained job keeps its sums
         if (n1.c == 0 && !chained) {
@@ -977,11 +1016,12 @@ struct Tile {
     // partial) with leading dimension ldo.  The buffer ends right after
     // element (M-1, N-1), so rows >= M fall past it; columns >= N (in range,
     // they belong to the next tile) take an out-of-range offset.
-    template <typename OutT>
-    __device__ __forceinline__ void store(OutT *__restrict__ out, int ldo, int M, int N) const
+    template <typename OutT, class Cols = PlainCols>
+    __device__ __forceinline__ void store(OutT *__restrict__ out, int ldo, int M, int N,
+                                          const Cols &cm = Cols{}) const
     {
         constexpr int ES_OUT = sizeof(OutT);
-        const __amdgpu_buffer_rsrc_t rs = make_rsrc(out, ((M - 1) * ldo + N) * ES_OUT);
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(out, cm.extent(M, ldo, N) * ES_OUT);
         const int lane = threadIdx.x & (kWave - 1);
         const int r = lane & 15, q = lane >> 4;
         const int rstep = (acc_row(0, 1) - acc_row(0, 0)) * ldo * ES_OUT;  // offset between a lane's rows
@@ -992,8 +1032,15 @@ struct Tile {
             for (int b = 0; b < WN; ++b) {
                 const int col = (wc0 + b) * 16 + r;
                 const int row = (wr0 + a) * 16 + acc_row(q, 0);
-                const int off = (row * ldo + col) * ES_OUT | (col < N ? 0 : kOOB);
-                if (scale) {
+                const int off = (row * ldo + cm(col)) * ES_OUT | (col < N ? 0 : kOOB);
+                if constexpr (Cols::MASK_ROWS) {
+                    // rows past M masked one by one (no alpha / beta here)
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const int ri = (wr0 + a) * 16 + acc_row(q, i);
+                        buf_store1<OutT>(rs, ri < M ? off + i * rstep : kOOB, (OutT)acc[a][b][i]);
+                    }
+                } else if (scale) {
                     OutT old[4] = {OutT(0), OutT(0), OutT(0), OutT(0)};
                     if (beta != T(0)) {
 #pragma unroll
@@ -1017,6 +1064,7 @@ struct ArraySrc {
     static constexpr int IM2COL = kPlainB;
     const GemmDesc *__restrict__ jobs;
     int n;
+    __device__ __forceinline__ PlainCols cols() const { return {}; }
     __device__ __forceinline__ int count() const { return n; }
     // The descriptor is read through the constant address space with a
     // uniform index, so it arrives by scalar loads (lgkmcnt), not vector
@@ -1036,6 +1084,7 @@ struct ArraySrc {
 struct OneSrc {
     static constexpr int IM2COL = kPlainB;
     GemmDesc job;
+    __device__ __forceinline__ PlainCols cols() const { return {}; }
     __device__ __forceinline__ int count() const { return 1; }
     __device__ __forceinline__ GemmDesc get(int) const { return job; }
 };

@@ -225,6 +225,26 @@ def test_conv_backward_layers_match_fp64(layer, prec):
     _assert_within(gw, rw, aw, batch * HW * HW, prec, f"dW {layer}")
 
 
+def test_conv_backward_batch_folded_few_channels():
+    """The input gradient of small images folds the batch into one GEMM
+    whose 16-row sub-tiles hold 2 live rows: the other rows' stores must be
+    dropped (they would land on other images' pixels).  Repeated calls, each
+    checked against the oracle and all bit-identical."""
+    g = (24, 2, 5, 5, 3, 3, 3, (1, 1), (1, 1), (1, 1))
+    batch, C, H, W, Cout, R, S, st, pad, dil = g
+    X, Wt = _inputs(batch, C, H, W, Cout, R, S, seed=41)
+    dY = np.random.default_rng(42).uniform(-1, 1, (batch, Cout, H, W)).astype(np.float32)
+    rx, ax, _, _ = oracle.sconv2d_bwd_ref(X, Wt, dY, st, pad, dil)
+    first = None
+    for rep in range(8):
+        (gx, tx), _ = _run_bwd(X, Wt, dY, st, pad, dil, ls.FP32)
+        _assert_within(gx, rx, ax, Cout * R * S, ls.FP32, f"dX {g} call {rep}")
+        assert np.isnan(tx).all(), "write past the end of dX"
+        if first is None:
+            first = gx
+        assert np.array_equal(gx, first), f"call {rep} differs from the first"
+
+
 def test_conv_backward_deterministic_and_single_gradients():
     """Sliced reductions summed in a fixed order: the same bits on every
     call, and each gradient alone equals the fused launch's."""
