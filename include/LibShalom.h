@@ -198,9 +198,8 @@ int LibShalom_sconv2d_forward(float *Y, const float *W, const float *X, long bat
  * batch, from dY and X.  Either output may be NULL to skip it.  dX is the
  * convolution of dY with the flipped, transposed filter (written by the
  * library into its per-stream workspace), fractionally strided for a
- * strided convolution; for stride > 1 it takes filters of at most 32 taps
- * and strides up to 8 (LIBSHALOM_NOT_SUPPORTED otherwise; dW takes any
- * geometry).  Split reductions
+ * strided convolution, with strides up to 8 (LIBSHALOM_NOT_SUPPORTED
+ * otherwise; dW takes any geometry).  Split reductions
  * are summed inside the launch in a fixed order.  Limits as for the forward,
  * with H + 2 pad_h (W + 2 pad_w) below 2^28, one dY image below 2 GiB, and
  * Cout*R*S and H*W also below 2^21. */

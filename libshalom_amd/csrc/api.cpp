@@ -1503,10 +1503,9 @@ int LibShalom_sconv2d_backward(float *dX, float *dWt, const float *dY, const flo
         K >= (double)kMaxLd || (double)Cout * R * S >= (double)kMaxLd || PQ >= (double)kMaxLd ||
         (double)H * W >= (double)kMaxLd)
         return set_status(LIBSHALOM_NOT_SUPPORTED);
-    // the input gradient of a strided convolution: filters of at most 32
-    // taps (its gather uses the tap mask) and strides up to 8 (exact
+    // the input gradient of a strided convolution: strides up to 8 (exact
     // multiply-high division of r*dil by the stride)
-    if (dX && (stride_h != 1 || stride_w != 1) && (R * S > 32 || stride_h > 8 || stride_w > 8))
+    if (dX && (stride_h > 8 || stride_w > 8))
         return set_status(LIBSHALOM_NOT_SUPPORTED);
     if (batch == 0 || (!dX && !dWt)) return set_status(LIBSHALOM_SUCCESS);
     if (!dY || (dX && !Wt) || (dWt && !X)) return set_status(LIBSHALOM_INVALID_POINTER);

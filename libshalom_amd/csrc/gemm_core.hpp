@@ -64,7 +64,7 @@ struct GemmDesc {
 // image is (C*R*S) x (P*Q): row t = (c*R + r)*S + s, column n = p*Q + q.
 //
 // Fractionally strided form (th or tw > 1; the input gradient of a strided
-// convolution, filter taps of at most 32): tap (r, s) of output pixel (p, q)
+// convolution, strides up to 8): tap (r, s) of output pixel (p, q)
 // reads X[c][(p*sh - ph + r*dh) / th][(q*sw - pw + s*dw) / tw] when both
 // divisions are exact, and contributes nothing otherwise.
 struct ConvGeom {
@@ -418,7 +418,9 @@ struct Stage {
     // tap (0, 0), its byte offset in channel 0, and -- for filters of at
     // most 32 taps -- the mask of taps (bit r*S + s) that fall inside the
     // image.  A row past the tile (or the panel) gets ih0 = 2^29, which no
-    // tap brings back into the image, and an empty mask.
+    // tap brings back into the image, and an empty mask.  A fractionally
+    // strided gather with more than 32 taps keeps the pixel's remainders
+    // al | al2 << 16 (below) in place of the mask.
     struct Pixels {
         int ih0[PER], iw0[PER], base[PER];
         unsigned taps[PER];
@@ -463,16 +465,20 @@ struct Stage {
                 const int a2 = floordiv(v, g.tw), al2 = v - a2 * g.tw;
                 ih0 = rok ? a + (al > 0) : (1 << 29);
                 iw0 = a2 + (al2 > 0);
-                unsigned cols = 0;
-                for (int s = 0; s < g.S; ++s) {
-                    const int b = s * g.dw / g.tw, be = s * g.dw - b * g.tw;
-                    const bool hit = (al2 + be == 0) | (al2 + be == g.tw);
-                    cols |= (unsigned)(hit & ((unsigned)(iw0 + b) < (unsigned)g.W)) << s;
-                }
-                for (int r = 0; r < g.R; ++r) {
-                    const int b = r * g.dh / g.th, be = r * g.dh - b * g.th;
-                    const bool hit = (al + be == 0) | (al + be == g.th);
-                    if (hit & ((unsigned)(ih0 + b) < (unsigned)g.H)) taps |= cols << (r * g.S);
+                if (g.RS <= 32) {
+                    unsigned cols = 0;
+                    for (int s = 0; s < g.S; ++s) {
+                        const int b = s * g.dw / g.tw, be = s * g.dw - b * g.tw;
+                        const bool hit = (al2 + be == 0) | (al2 + be == g.tw);
+                        cols |= (unsigned)(hit & ((unsigned)(iw0 + b) < (unsigned)g.W)) << s;
+                    }
+                    for (int r = 0; r < g.R; ++r) {
+                        const int b = r * g.dh / g.th, be = r * g.dh - b * g.th;
+                        const bool hit = (al + be == 0) | (al + be == g.th);
+                        if (hit & ((unsigned)(ih0 + b) < (unsigned)g.H)) taps |= cols << (r * g.S);
+                    }
+                } else {
+                    taps = (unsigned)al | ((unsigned)al2 << 16);
                 }
             }
             // modulo 2^32: only used where ih0 is a real row; kept opaque so
@@ -499,7 +505,8 @@ struct Stage {
     // lanes at once) and the wave reads the decoded taps it holds from those
     // lanes (v_readlane), so per element a filter of at most 32 taps costs a
     // test of the tap's bit in the pixel's mask, a larger one two adds and
-    // two range compares.  (Decoding in scalar registers made the CU's one
+    // two range compares (and, fractionally strided, the test that both
+    // remainders sum to 0 or the stride).  (Decoding in scalar registers made the CU's one
     // scalar unit the bottleneck.)  The offsets are formed first and the
     // loads issued after, outside the branch on the filter size.
     __device__ __forceinline__ void load_im2col(const T *__restrict__ X, const ConvGeom &g, const Pixels &px,
@@ -534,6 +541,32 @@ struct Stage {
                     const unsigned bit = __builtin_amdgcn_readlane(lbit, kw + e);
                     const int toff = __builtin_amdgcn_readlane(ltoff, kw + e);
                     off[i][e] = (px.taps[i] & bit) ? px.base[i] + toff : kOOB;
+                }
+            }
+        } else if (g.th != 1 || g.tw != 1) {
+            // fractionally strided, more than 32 taps: the tap's row and
+            // column parts b (tdy, tdx) and remainders be, the pixel's
+            // remainders al in px.taps (al | al2 << 16, each below the
+            // stride, so the packed sum keeps the halves apart); the tap
+            // contributes when al + be is 0 or the stride in both
+            // directions.  The row sentinel is as below.
+            const int ldy = tl < kend ? tdy : (1 << 29), ldx = tdx;
+            const int lbe = (tr * g.dh - tdy * g.th) | ((ts * g.dw - tdx * g.tw) << 16);
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                int row, k;
+                coords(tid + i * NT, row, k);
+                const int kw = uniform(k);
+#pragma unroll
+                for (int e = 0; e < VEC; ++e) {
+                    const int dy = __builtin_amdgcn_readlane(ldy, kw + e), dx = __builtin_amdgcn_readlane(ldx, kw + e);
+                    const int toff = __builtin_amdgcn_readlane(ltoff, kw + e);
+                    const int sum = (int)px.taps[i] + __builtin_amdgcn_readlane(lbe, kw + e);
+                    const int sy = sum & 0xffff, sx = sum >> 16;
+                    const bool ok = ((unsigned)(px.ih0[i] + dy) < (unsigned)g.H) &
+                                    ((unsigned)(px.iw0[i] + dx) < (unsigned)g.W) & ((sy == 0) | (sy == g.th)) &
+                                    ((sx == 0) | (sx == g.tw));
+                    off[i][e] = ok ? px.base[i] + toff : kOOB;
                 }
             }
         } else {

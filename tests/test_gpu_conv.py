@@ -45,6 +45,8 @@ GEOMS = [
     (1, 6, 12, 12, 9, 4, 4, (2, 2), (1, 1), (1, 1)),     # 4 x 4, stride 2
     (1, 3, 11, 10, 4, 3, 3, (3, 3), (1, 0), (1, 1)),     # stride 3
     (1, 3, 16, 16, 4, 5, 5, (4, 2), (2, 2), (1, 1)),     # stride (4, 2), 5 x 5
+    (2, 3, 14, 13, 5, 6, 6, (3, 2), (2, 1), (1, 2)),     # R*S = 36, stride (3, 2), dilation (1, 2)
+    (2, 2, 23, 21, 3, 7, 7, (2, 2), (3, 3), (1, 1)),     # the 7 x 7 stride-2 stem, odd sizes
 ]
 
 
@@ -149,13 +151,10 @@ def test_conv_empty_batch_is_a_no_op():
 
 
 # Backward: stride-1 geometries, and strided ones (dX through the fractionally
-# strided gather for filters of at most 32 taps; dW alone beyond that).
+# strided gather: the per-tile tap mask for filters of at most 32 taps, the
+# per-element remainder test beyond).
 BWD_GEOMS = [g for g in GEOMS if g[7] == (1, 1)]
 BWD_STRIDED = [g for g in GEOMS if g[7] != (1, 1)]
-
-
-def _strided_dx_supported(g):
-    return g[5] * g[6] <= 32
 
 
 def _run_bwd(X, Wt, dY, st, pad, dil, prec, want_dx=True, guard=64):
@@ -196,7 +195,7 @@ def test_conv_backward_matches_oracle(g, prec):
 @pytest.mark.parametrize("g", BWD_STRIDED)
 def test_conv_backward_strided(g, prec):
     print(f"conv backward (strided) {g} precision {prec}", flush=True)
-    _bwd_case(g, prec, _strided_dx_supported(g))
+    _bwd_case(g, prec, True)
 
 
 def _torch_grads(X, Wt, dY):
