@@ -288,8 +288,9 @@ def sconv2d_forward(Y, Wt, X, batch, C, H, W, Cout, R, S, stride=(1, 1), padding
 
 def sconv2d_backward(dX, dW, dY, Wt, X, batch, C, H, W, Cout, R, S, stride=(1, 1), padding=(0, 0),
                      dilation=(1, 1), precision=FP32, stream=None):
-    """dX and dW of sconv2d_forward in one launch (either may be None to skip it; dX needs
-    stride 1); the im2col matrix is gathered, never stored."""
+    """dX and dW of sconv2d_forward in one launch (either may be None to skip it; dX takes
+    strides up to 8, fractionally strided beyond 1); the im2col matrix is gathered, never
+    stored."""
     _check(_lib.LibShalom_sconv2d_backward(_ptr(dX), _ptr(dW), _ptr(dY), _ptr(Wt), _ptr(X), batch, C, H, W,
                                            Cout, R, S, stride[0], stride[1], padding[0], padding[1],
                                            dilation[0], dilation[1], precision, _stream(stream)),
