@@ -569,7 +569,7 @@ void fold_into(BwdProblem &P, ConvGeom &g, long batch, long pixels, long in_img,
 }
 
 // Wf[c][o][R-1-r][S-1-s] = W[o][c][r][s]: the filter of the input gradient
-// of a stride-1 convolution, as a C x Cout*R*S matrix.
+// (any stride), as a C x Cout*R*S matrix.
 __global__ void k_conv_flip(const float *__restrict__ Wt, float *__restrict__ Wf, int Cout, int C, int RS)
 {
     const long n = (long)Cout * C * RS;

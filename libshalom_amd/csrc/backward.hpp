@@ -244,9 +244,9 @@ k_conv_fwd(BwdProblem P, ConvGeom g)
 }
 
 // Convolution backward as two implicit GEMMs in ONE launch.  X units: the
-// input gradient of a stride-1 convolution, itself a convolution of dY
-// (geometry gx) with the filter flipped and transposed (C x Cout*R*S, in
-// the problem's left operand).  Y units: the filter gradient
+// input gradient, itself a convolution of dY (geometry gx; fractionally
+// strided for a strided convolution) with the filter flipped and transposed
+// (C x Cout*R*S, in the problem's left operand).  Y units: the filter gradient
 // dW (Cout x C*R*S) = sum over images of dY_i (Cout x P*Q) times the
 // transposed im2col matrix of X_i, gathered from the image (geometry gy),
 // the sum over images and pixels cut into slices as for a shared operand.
