@@ -630,7 +630,11 @@ int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *W
     for (int i = 0; i < n; ++i) mind = std::min(mind, (long)std::min(probs[i]->R, probs[i]->Cc));
     const int T = plan_problems(probs, n, mind >= 128, dev, info.num_cu, s, prec == P_FP32 ? 3 : 2);
     if (T <= 0) return -T;
-    ConvBwdLaunch fn = bwd_table().conv_bwd[prec][T == 128];
+    // dW's gather in 16-byte runs when a thread's four pixels (slice starts
+    // are multiples of 32) always share an output row and read consecutive
+    // input columns
+    const bool runs = sw == 1 && Q % 4 == 0;
+    ConvBwdLaunch fn = bwd_table().conv_bwd[prec][T == 128][runs];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
     return fn(X_, Y_, gx, gy, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }

@@ -174,7 +174,7 @@ __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, 
     // a convolution slice starts on a channel (slices are multiples of R*S)
     if constexpr (IM2COL == kIm2col)
         src.boff = (long long)(src.rho0 / geom.RS) * geom.H * geom.W;
-    else if constexpr (IM2COL == kIm2colT)
+    else if constexpr (IM2COL == kIm2colT || IM2COL == kIm2colTR)
         src.boff = 0;
     else
         src.boff = Cfg::TB ? (long long)src.c0 * P.ldb + src.rho0 : (long long)src.rho0 * P.ldb + src.c0;
@@ -250,7 +250,7 @@ k_conv_fwd(BwdProblem P, ConvGeom g)
 // dW (Cout x C*R*S) = sum over images of dY_i (Cout x P*Q) times the
 // transposed im2col matrix of X_i, gathered from the image (geometry gy),
 // the sum over images and pixels cut into slices as for a shared operand.
-template <class CX, class CY>
+template <class CX, class CY, int IMY>
 __global__ __attribute__((amdgpu_flat_work_group_size(CX::NT, CX::NT))) void
 k_conv_backward(BwdProblem X, BwdProblem Y, ConvGeom gx, ConvGeom gy)
 {
@@ -260,7 +260,7 @@ k_conv_backward(BwdProblem X, BwdProblem Y, ConvGeom gx, ConvGeom gy)
     if (u < X.units)
         bwd_unit<CX, kIm2col>(X, u, lds, &flag, gx);
     else
-        bwd_unit<CY, kIm2colT>(Y, u - X.units, lds, &flag, gy);
+        bwd_unit<CY, IMY>(Y, u - X.units, lds, &flag, gy);
 }
 
 // ---------------------------------------------------------------------------
@@ -272,7 +272,8 @@ typedef hipError_t (*ConvLaunch)(const BwdProblem &P, const ConvGeom &g, hipStre
 typedef hipError_t (*ConvBwdLaunch)(const BwdProblem &X, const BwdProblem &Y, const ConvGeom &gx,
                                     const ConvGeom &gy, hipStream_t s);
 
-template <int PREC, int BT>
+// RUNS: dW's gather in 16-byte runs (stride 1 along W, Q a multiple of 4).
+template <int PREC, int BT, bool RUNS>
 hipError_t launch_conv_backward(const BwdProblem &X, const BwdProblem &Y, const ConvGeom &gx, const ConvGeom &gy,
                                 hipStream_t s)
 {
@@ -281,7 +282,8 @@ hipError_t launch_conv_backward(const BwdProblem &X, const BwdProblem &Y, const 
     static_assert(CX::NT == CY::NT, "one workgroup size for both problems");
     const int grid = X.units + Y.units;
     if (grid <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_conv_backward<CX, CY>), dim3(grid), dim3(CX::NT), 0, s, X, Y, gx, gy);
+    hipLaunchKernelGGL((k_conv_backward<CX, CY, RUNS ? kIm2colTR : kIm2colT>), dim3(grid), dim3(CX::NT), 0, s, X, Y,
+                       gx, gy);
     return hipGetLastError();
 }
 
@@ -325,7 +327,7 @@ struct BwdTable {
     BwdLaunch fn[NPREC][2][2][NBWD_TILE];
     StridedLaunch strided[NPREC][2][2][NBWD_TILE];
     ConvLaunch conv_fwd[NPREC][NBWD_TILE];
-    ConvBwdLaunch conv_bwd[NPREC][NBWD_TILE];
+    ConvBwdLaunch conv_bwd[NPREC][NBWD_TILE][2];  // [.][.][dW in 16-byte runs]
 };
 BwdTable &bwd_table();
 
@@ -350,8 +352,10 @@ void register_backward(BwdTable &t)
     t.strided[PREC][1][1][1] = &launch_strided<PREC, 1, 1, 128>;
     t.conv_fwd[PREC][0] = &launch_conv_fwd<PREC, 64>;
     t.conv_fwd[PREC][1] = &launch_conv_fwd<PREC, 128>;
-    t.conv_bwd[PREC][0] = &launch_conv_backward<PREC, 64>;
-    t.conv_bwd[PREC][1] = &launch_conv_backward<PREC, 128>;
+    t.conv_bwd[PREC][0][0] = &launch_conv_backward<PREC, 64, false>;
+    t.conv_bwd[PREC][1][0] = &launch_conv_backward<PREC, 128, false>;
+    t.conv_bwd[PREC][0][1] = &launch_conv_backward<PREC, 64, true>;
+    t.conv_bwd[PREC][1][1] = &launch_conv_backward<PREC, 128, true>;
 }
 
 void register_backward_fp32(BwdTable &t);

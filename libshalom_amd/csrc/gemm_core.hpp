@@ -88,8 +88,11 @@ __device__ __forceinline__ int floordiv(int x, int d) { return x >= 0 ? x / d : 
 // How a job source's B operand is formed: read from memory; gathered as a
 // convolution's im2col matrix (k = filter tap, column = output pixel); or,
 // for a B stored transposed, gathered as the im2col matrix itself (stored
-// row = filter tap, k = output pixel).
-enum BSource : int { kPlainB = 0, kIm2col = 1, kIm2colT = 2 };
+// row = filter tap, k = output pixel).  kIm2colTR is kIm2colT for a
+// convolution of stride 1 along W whose output width Q is a multiple of 4:
+// a thread's four consecutive pixels then share one output row and read
+// four consecutive input columns, taken by one 16-byte load.
+enum BSource : int { kPlainB = 0, kIm2col = 1, kIm2colT = 2, kIm2colTR = 3 };
 
 // x / d for the multiplier m = ceil(2^32 / d) (m == 0: d == 1).  Exact when
 // x * d < 2^32: with e = m*d - 2^32 < d, x*m / 2^32 = x/d + x*e / (d 2^32),
@@ -361,6 +364,7 @@ struct Stage {
     static constexpr int ES = sizeof(T);
     T v[PER][VEC];
     int kcut;  // valid k of the chunk (k-contiguous panels mask the rest at the LDS write)
+    int run[PER];  // 16-byte im2col runs: leading padding lo | valid end hi << 8
 
     // First element (row, k) of the thread's group of VEC elements.
     __device__ __forceinline__ static void coords(int idx, int &row, int &k)
@@ -631,6 +635,16 @@ struct Stage {
     // kend pixels.  Each thread holds VEC consecutive pixels of one tap:
     // the first is decoded (one division), the others follow by stepping
     // along the output row.
+    //
+    // RUNS (stride 1 along W, Q and n0 multiples of VEC): the VEC pixels
+    // share one output row, so the thread loads the VEC input columns of the
+    // tap with one 16-byte load that starts at the first one inside the
+    // image (never before the channel's start), and fix_runs() moves them
+    // into place once they have landed: lo leading columns in the left
+    // padding are zero, and so is everything from hi on (right padding or
+    // the chunk's end).  The load's columns past the row belong to the next
+    // row or channel and are among those zeroed.
+    template <bool RUNS = false>
     __device__ __forceinline__ void load_im2col_t(const T *__restrict__ X, const ConvGeom &g, const Taps &tp, int n0,
                                                   int kend, int k0, bool live, int tid)
     {
@@ -639,6 +653,25 @@ struct Stage {
         kcut = kc;
         const int HW = g.H * g.W;
         const __amdgpu_buffer_rsrc_t rx = make_rsrc(X, (live && kc > 0) ? g.C * HW * ES : 0);
+        if constexpr (RUNS) {
+            static_assert(VEC == 4, "runs of four 4-byte pixels");
+#pragma unroll
+            for (int i = 0; i < PER; ++i) {
+                int row, k;
+                coords(tid + i * NT, row, k);
+                const int n = n0 + k0 + k;
+                const int p = udiv(n, g.mQ), q = n - p * g.Q;
+                const int ih = p * g.sh - g.ph, iw = q - g.pw;
+                const int iwt = iw + tp.dx[i];  // the tap's first input column
+                const int lo = min(max(-iwt, 0), VEC), hi = max(min(min(g.W - iwt, kc - k), VEC), 0);
+                const bool ok = ((unsigned)(ih + tp.dy[i]) < (unsigned)g.H) & (lo < hi);
+                run[i] = lo | (hi << 8);
+                // byte offset of column iwt + lo >= 0 of the tap's row
+                const unsigned o = (unsigned)tp.toff[i] + ((unsigned)ih * (unsigned)g.W + (unsigned)(iw + lo)) * ES;
+                buf_load16<T>(rx, ok ? (int)o : kOOB, v[i]);
+            }
+            return;
+        }
         const unsigned step_col = g.sw * ES, step_row = (unsigned)(g.sh * g.W - (g.Q - 1) * g.sw) * ES;
 #pragma unroll
         for (int i = 0; i < PER; ++i) {
@@ -661,6 +694,26 @@ struct Stage {
                 iw = wrap ? -g.pw : iw + g.sw;
                 pix += wrap ? step_row : step_col;
             }
+        }
+    }
+
+    // Places the columns of 16-byte im2col runs once the loads have landed:
+    // element e is loaded column e - lo for lo <= e < hi, else zero.
+    __device__ __forceinline__ void fix_runs()
+    {
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int lo = run[i] & 255, hi = run[i] >> 8;
+            T w[VEC];
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) {
+                T x = v[i][e];
+#pragma unroll
+                for (int s = 1; s <= e; ++s) x = lo == s ? v[i][e - s] : x;
+                w[e] = (e >= lo) & (e < hi) ? x : T(0);
+            }
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) v[i][e] = w[e];
         }
     }
 
@@ -743,7 +796,7 @@ struct Tile {
     {
         if constexpr (Src::IM2COL == kIm2col)
             return StageBI::pixels(src.geom, d0.ldb, d0.N, tid);
-        else if constexpr (Src::IM2COL == kIm2colT)
+        else if constexpr (Src::IM2COL == kIm2colT || Src::IM2COL == kIm2colTR)
             return StageB::taps(src.geom, src.c0, d0.N, tid);
         else
             return NoLanes{};
@@ -909,6 +962,8 @@ struct Tile {
             sb.load_im2col((const T *)d.B, src.geom, px, d.K, k0, live, tid);
         else if constexpr (Src::IM2COL == kIm2colT)
             sb.load_im2col_t((const T *)d.B, src.geom, px, src.rho0, d.K, k0, live, tid);
+        else if constexpr (Src::IM2COL == kIm2colTR)
+            sb.template load_im2col_t<true>((const T *)d.B, src.geom, px, src.rho0, d.K, k0, live, tid);
         else
             sb.load((const T *)d.B, d.ldb, d.N, d.K, k0, live, tid);
     }
@@ -982,6 +1037,7 @@ struct Tile {
         // make it wait for n2's loads as well and drain the prefetch at every
         // job end.
         if constexpr (Cfg::NBUF == 1) __syncthreads();
+        if constexpr (Src::IM2COL == kIm2colTR) b1.fix_runs();
         chunk_to_lds(a1, b1, lds_buf(lds, bi + 1), tid);
         if (cur.c + 1 == cur.nch && !chained) store((T *)dc.C, dc.ldc, dc.M, dc.N, src.cols());
         if (n1.job < 0) return false;
@@ -1025,6 +1081,7 @@ struct Tile {
         load_chunk(a0, b0, src, px, d0, 0, true, tid);
         load_chunk(a1, b1, src, px, src.get(max(n1.job, 0)), n1.c * BK, n1.job >= 0, tid);
         begin(d0.M, d0.N, 0, 0, wave);
+        if constexpr (Src::IM2COL == kIm2colTR) b0.fix_runs();
         chunk_to_lds(a0, b0, lds_buf(lds, 0), tid);
         __syncthreads();
         for (;;) {

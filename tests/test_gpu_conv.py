@@ -47,6 +47,10 @@ GEOMS = [
     (1, 3, 16, 16, 4, 5, 5, (4, 2), (2, 2), (1, 1)),     # stride (4, 2), 5 x 5
     (2, 3, 14, 13, 5, 6, 6, (3, 2), (2, 1), (1, 2)),     # R*S = 36, stride (3, 2), dilation (1, 2)
     (2, 2, 23, 21, 3, 7, 7, (2, 2), (3, 3), (1, 1)),     # the 7 x 7 stride-2 stem, odd sizes
+    # Q a multiple of 4 and stride 1 along W: dW's gather in 16-byte runs
+    (2, 3, 10, 12, 4, 3, 7, (1, 1), (1, 3), (1, 1)),     # up to 3 leading padding columns per run
+    (1, 2, 9, 16, 3, 2, 3, (1, 1), (0, 2), (1, 2)),      # dilation 2 along W
+    (2, 3, 11, 8, 5, 3, 3, (2, 1), (1, 1), (1, 1)),      # stride 2 along H only
 ]
 
 
