@@ -119,8 +119,18 @@ __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_o
     for (int e = threadIdx.x * 4; e < tr * BN; e += NT * 4) {
         const int r = e / BN, c = e % BN;
         if (c >= tcn) continue;
+        // slices summed in slice order; eight loads in flight at a time (a
+        // load-then-add loop put one memory round trip per slice in series)
         f32x4 acc = *(const f32x4 *)(base + e);
-        for (int s = 1; s < P.nslices; ++s) acc += *(const f32x4 *)(base + (size_t)s * (BM * BN) + e);
+        int s = 1;
+        for (; s + 8 <= P.nslices; s += 8) {
+            f32x4 t[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = *(const f32x4 *)(base + (size_t)(s + j) * (BM * BN) + e);
+#pragma unroll
+            for (int j = 0; j < 8; ++j) acc += t[j];
+        }
+        for (; s < P.nslices; ++s) acc += *(const f32x4 *)(base + (size_t)s * (BM * BN) + e);
         float *o = O + (long long)(r0 + r) * P.ldo;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {

@@ -710,7 +710,7 @@ struct Stage {
                 T x = v[i][e];
 #pragma unroll
                 for (int s = 1; s <= e; ++s) x = lo == s ? v[i][e - s] : x;
-                w[e] = (e >= lo) & (e < hi) ? x : T(0);
+                w[e] = ((e >= lo) & (e < hi)) ? x : T(0);
             }
 #pragma unroll
             for (int e = 0; e < VEC; ++e) v[i][e] = w[e];
