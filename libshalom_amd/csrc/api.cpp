@@ -397,9 +397,12 @@ void bwd_slice(BwdProblem &P, int T, long len)
 // slower, FP32 by 1.5x -- but its implicit backward, whose dW problem adds
 // tiles, 1.2x faster in BF16X3 (and 7% slower in FP32), so that launch plans
 // with one half for the split-BF16 modes.)  A reduction is cut into
-// slices of about W / (2 * CUs) (W = all tiles' reduction work) so that no
-// unit is much longer than a compute unit's share; slices cost partial-tile
-// traffic, which is kept under a quarter of the operand traffic.  Returns
+// slices of about W / (2 * CUs) for 64 x 64 tiles, W / CUs for 128 x 128
+// tiles (W = all tiles' reduction work) so that no unit is much longer than
+// a compute unit's share: a 128 x 128 workgroup fills its CU alone, and one
+// unit per CU there measured neutral to 1.4x faster than two on the conv
+// layers' backward launches; slices cost partial-tile traffic, which is
+// kept under a quarter of the operand traffic.  Returns
 // the tile size (0: nothing to launch, < 0: -status).
 int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu, hipStream_t s,
                   int big_quarters = 3)
@@ -422,7 +425,7 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
         ops += 4.0 * ((P.sL == 0 ? 1.0 : P.nimg) * P.R * P.red + (P.sR == 0 ? 1.0 : P.nimg) * P.red * P.Cc +
                       outs * P.R * P.Cc);
     }
-    long len = std::max(256L, (long)std::ceil(W / (2.0 * num_cu)));
+    long len = std::max(256L, (long)std::ceil(W / ((T == 128 ? 1.0 : 2.0) * num_cu)));
     len = ceil_div(len, 32) * 32;
     size_t slot_bytes = 0, counters = 0;
     long units = 0;
