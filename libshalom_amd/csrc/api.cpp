@@ -425,26 +425,53 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
         ops += 4.0 * ((P.sL == 0 ? 1.0 : P.nimg) * P.R * P.red + (P.sR == 0 ? 1.0 : P.nimg) * P.red * P.Cc +
                       outs * P.R * P.Cc);
     }
-    long len = std::max(256L, (long)std::ceil(W / ((T == 128 ? 1.0 : 2.0) * num_cu)));
-    len = ceil_div(len, 32) * 32;
+    long len0 = std::max(256L, (long)std::ceil(W / ((T == 128 ? 1.0 : 2.0) * num_cu)));
+    len0 = ceil_div(len0, 32) * 32;
     size_t slot_bytes = 0, counters = 0;
     long units = 0;
-    for (int it = 0; it < 40; ++it) {
-        slot_bytes = counters = 0;
-        units = 0;
-        for (int i = 0; i < n; ++i) {
-            BwdProblem &P = *probs[i];
-            bwd_slice(P, T, len);
-            units += P.units;
-            if (P.nslices > 1) {
-                slot_bytes += (size_t)P.units * T * T * sizeof(float);
-                counters += (size_t)(P.sO == 0 ? 1 : P.nimg) * P.ntiles;
+    // Slices every problem at slice length len, doubled until the partial
+    // tiles fit their limits.
+    auto slice_all = [&](long len) {
+        for (int it = 0; it < 40; ++it) {
+            slot_bytes = counters = 0;
+            units = 0;
+            for (int i = 0; i < n; ++i) {
+                BwdProblem &P = *probs[i];
+                bwd_slice(P, T, len);
+                units += P.units;
+                if (P.nslices > 1) {
+                    slot_bytes += (size_t)P.units * T * T * sizeof(float);
+                    counters += (size_t)(P.sO == 0 ? 1 : P.nimg) * P.ntiles;
+                }
             }
+            const bool traffic_ok = 2.0 * slot_bytes <= 0.25 * ops;
+            if (traffic_ok && slot_bytes <= ((size_t)256 << 20) && counters * sizeof(int) <= kCounterBytes) break;
+            len *= 2;
         }
-        const bool traffic_ok = 2.0 * slot_bytes <= 0.25 * ops;
-        if (traffic_ok && slot_bytes <= ((size_t)256 << 20) && counters * sizeof(int) <= kCounterBytes) break;
-        len *= 2;
+    };
+    // Of 1, 2 and 4 times that length, the one with the least estimated
+    // makespan: rounds of units over the resident workgroups (one per CU
+    // for 128 x 128 tiles, two for 64 x 64) times the longest unit's
+    // reduction steps; ties go to the longer slices (fewer partial tiles).
+    // (The 7x7 conv layer's backward, 300 units of 128 x 128 for 256 CUs at
+    // the base length, runs in one round of 248 at twice it: 0.21 -> 0.15 ms.)
+    const long resident = (long)num_cu * (T == 128 ? 1 : 2);
+    long best = len0;
+    double best_cost = 0;
+    for (int m = 1; m <= 4; m *= 2) {
+        slice_all(len0 * m);
+        double longest = 0;
+        for (int i = 0; i < n; ++i) {
+            const BwdProblem &P = *probs[i];
+            if (P.units > 0) longest = std::max(longest, P.spi > 0 ? (double)P.slen : (double)P.gimg * P.red);
+        }
+        const double cost = (double)ceil_div(units, resident) * longest;
+        if (m == 1 || cost <= best_cost) {
+            best = len0 * m;
+            best_cost = cost;
+        }
     }
+    slice_all(best);
     if (units > INT32_MAX) return -LIBSHALOM_NOT_SUPPORTED;
     if (units == 0) return 0;
     if (counters == 0) return T;
