@@ -214,10 +214,12 @@ def _torch_grads(X, Wt, dY):
 
 @pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
 @pytest.mark.parametrize("layer", [(64, 64, 56, 2), (128, 128, 28, 2), (256, 256, 14, 2), (512, 512, 7, 2),
-                                   (256, 16, 32, 16)])
+                                   (256, 16, 32, 16), (512, 512, 7, 32)])
 def test_conv_backward_layers_match_fp64(layer, prec):
     """The 3 x 3 layers against torch's float64 autograd (the |.| products
-    for the bound are the gradients of the |.| problem)."""
+    for the bound are the gradients of the |.| problem).  The 7 x 7 layer at
+    batch 32 takes the plan whose slices are twice the base length (one round
+    of 248 units of 128 x 128 on 256 compute units)."""
     Cout, C, HW, batch = layer
     print(f"conv backward layer {layer} precision {prec}", flush=True)
     X, Wt = _inputs(batch, C, HW, HW, Cout, 3, 3, seed=Cout + C + 1)
