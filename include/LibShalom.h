@@ -47,7 +47,9 @@ typedef void *LibShalom_stream_t;
 /* Arithmetic of the float GEMMs.  FP32: FP32 MFMA.  BF16X3: each FP32
  * operand split as hi + lo (bf16), three BF16 MFMA products hi*hi + hi*lo +
  * lo*hi, FP32 accumulation.  BF16: one BF16 product.  BF16X6: three-way
- * split, six products (about FP32 accuracy).  FP64 GEMMs always use FP64 MFMA. */
+ * split (exact for every FP32 value), the eight products of the pieces but
+ * lo2*lo2, FP32 accumulation: within |C - C64| <= 2*K*2^-24*(|A||B|), as
+ * FP32.  FP64 GEMMs always use FP64 MFMA. */
 #define LIBSHALOM_FP32 0
 #define LIBSHALOM_BF16X3 1
 #define LIBSHALOM_BF16 2

@@ -883,10 +883,14 @@ struct Tile {
 #pragma unroll
                 for (int pl = 0; pl < NP; ++pl)
                     af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + ((wr0 + a) * 16 + r) * LDKH + swz_k(r, q * 8));
-            // partial products (A piece, B piece), smallest first
-            constexpr int NPROD = NP == 1 ? 1 : (NP == 2 ? 3 : 6);
-            constexpr int PA[6] = {NP == 3 ? 2 : 1, NP == 3 ? 1 : 0, NP == 3 ? 0 : 0, 1, 0, 0};
-            constexpr int PB[6] = {0, NP == 3 ? 1 : 1, NP == 3 ? 2 : 0, 0, 1, 0};
+            // partial products (A piece, B piece), smallest first.  Two
+            // pieces: lo*hi, hi*lo, hi*hi.  Three pieces (exact for every
+            // FP32 value): all but lo2*lo2 (~2^-32 |a||b|) -- the two
+            // products of size ~2^-24 |a||b| are needed for the FP32 bound
+            // at K = 1, where one rounding already costs up to 2^-24.
+            constexpr int NPROD = NP == 1 ? 1 : (NP == 2 ? 3 : 8);
+            constexpr int PA[8] = {NP == 3 ? 2 : 1, NP == 3 ? 1 : 0, NP == 3 ? 2 : 0, 1, 0, 1, 0, 0};
+            constexpr int PB[8] = {NP == 3 ? 1 : 0, NP == 3 ? 2 : 1, 0, 1, 2, 0, 1, 0};
 #pragma unroll
             for (int b = 0; b < WN; ++b) {
                 bf16x8 bfr[NP];

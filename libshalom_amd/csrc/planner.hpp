@@ -49,7 +49,7 @@ inline double subtile_chunk_cycles(int prec)
     case P_FP32: return 8 * 32.0;   // 8 x v_mfma_f32_16x16x4_f32
     case P_BF16: return 16.0;       // 1 x v_mfma_f32_16x16x32_bf16
     case P_BF16X3: return 3 * 16.0;
-    case P_BF16X6: return 6 * 16.0;
+    case P_BF16X6: return 8 * 16.0;
     default: return 4 * 64.0;       // 4 x v_mfma_f64_16x16x4_f64
     }
 }

@@ -20,7 +20,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 U32 = 2.0 ** -24
-EXTRA = {ls.FP32: 0.0, ls.BF16X6: 4 * 2.0 ** -24, ls.BF16X3: 3 * 2.0 ** -16, ls.BF16: 2 * 2.0 ** -8}
+EXTRA = {ls.FP32: 0.0, ls.BF16X6: 0.0, ls.BF16X3: 3 * 2.0 ** -16, ls.BF16: 2 * 2.0 ** -8}
 TRANS = [(0, 0), (0, 1), (1, 0), (1, 1)]
 
 
