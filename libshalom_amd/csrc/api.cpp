@@ -59,6 +59,10 @@ BwdTable &bwd_table()
     }();
     return *t;
 }
+__global__ void k_xcc_probe(int *out, int n)
+{
+    if (threadIdx.x == 0 && (int)blockIdx.x < n) out[blockIdx.x] = xcc_id();
+}
 }  // namespace ls
 
 namespace {
@@ -81,9 +85,36 @@ struct DeviceInfo {
     bool checked = false;
     bool ok = false;
     int num_cu = 256;
+    // workgroup L of a 1-D launch runs on XCD L % 8 (checked once by
+    // k_xcc_probe): split-K slices of a tile can then share one L2
+    bool xcd_rr = false;
 };
 std::mutex g_dev_mu;
 std::map<int, DeviceInfo> g_devices;
+
+constexpr int kProbeBlocks = 64;
+__device__ int g_xcc_probe[kProbeBlocks];
+
+// Whether the device deals the workgroups of a 1-D launch to its 8 XCDs
+// round-robin: a launch of kProbeBlocks workgroups records each one's XCD.
+// Any failure (or a stream capture in progress) answers no.
+bool probe_xcd_round_robin()
+{
+    int *p = nullptr;
+    if (hipGetSymbolAddress((void **)&p, HIP_SYMBOL(g_xcc_probe)) != hipSuccess || !p) {
+        (void)hipGetLastError();
+        return false;
+    }
+    hipLaunchKernelGGL(k_xcc_probe, dim3(kProbeBlocks), dim3(64), 0, nullptr, p, kProbeBlocks);
+    int h[kProbeBlocks];
+    if (hipGetLastError() != hipSuccess || hipMemcpy(h, p, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    for (int i = 0; i < kProbeBlocks; ++i)
+        if (h[i] != i % 8) return false;
+    return true;
+}
 
 int current_device(int *dev, DeviceInfo *info)
 {
@@ -100,6 +131,7 @@ int current_device(int *dev, DeviceInfo *info)
         if (hipGetDeviceProperties(&prop, d) == hipSuccess) {
             di.ok = std::strncmp(prop.gcnArchName, "gfx950", 6) == 0;
             di.num_cu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+            if (di.ok) di.xcd_rr = probe_xcd_round_robin();
         } else {
             (void)hipGetLastError();
         }
@@ -159,6 +191,59 @@ int workspace(int dev, hipStream_t s, size_t bytes, void **out)
 }
 
 // ---------------------------------------------------------------------------
+// In-launch reductions (split-K of single GEMMs, sliced backward / strided
+// problems).
+// ---------------------------------------------------------------------------
+// Per (device, stream) memory for the split-reduction fixup: arrival
+// counters (zero between launches; every launch leaves them zero) and the
+// partial-tile slots.
+struct FixupSpace {
+    char *ptr = nullptr;
+    size_t size = 0;
+};
+std::mutex g_fx_mu;
+std::map<std::pair<int, hipStream_t>, FixupSpace> g_fx;
+constexpr size_t kCounterBytes = 1 << 20;  // 256 K tile counters
+// Partial tiles a single GEMM's split-K may use (the planner picks fewer K
+// slices when more would be needed).
+constexpr size_t kSplitKSlotLimit = (size_t)256 << 20;
+
+__global__ void k_zero_ints(int *p, int n)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+int fixup_space(int dev, hipStream_t s, size_t slot_bytes, int **counters, float **slots)
+{
+    if (const char *f = std::getenv("LIBSHALOM_FAIL_WORKSPACE"))
+        if (f[0] == '1') return LIBSHALOM_ALLOC_FAILED;  // failure injection for tests
+    std::lock_guard<std::mutex> lk(g_fx_mu);
+    FixupSpace &w = g_fx[{dev, s}];
+    const size_t need = kCounterBytes + slot_bytes;
+    if (w.size < need) {
+        if (w.ptr) {
+            if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
+            (void)hipFree(w.ptr);
+            w.ptr = nullptr;
+            w.size = 0;
+        }
+        const size_t sz = std::max(need, (size_t)8 << 20);
+        if (hipMalloc((void **)&w.ptr, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            w.ptr = nullptr;
+            return LIBSHALOM_ALLOC_FAILED;
+        }
+        w.size = sz;
+        // a fill kernel rather than a memset, so that a captured call records a kernel node
+        hipLaunchKernelGGL(k_zero_ints, dim3(256), dim3(256), 0, s, (int *)w.ptr, (int)(kCounterBytes / sizeof(int)));
+        if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
+    }
+    *counters = (int *)w.ptr;
+    *slots = (float *)(w.ptr + kCounterBytes);
+    return LIBSHALOM_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
 // Argument checks shared by every entry point.
 // ---------------------------------------------------------------------------
 int check_dims(int ta, int tb, long M, long N, long K)
@@ -213,6 +298,29 @@ int check_lds(int ta, int tb, long M, long N, long K, long lda, long ldb, long l
     return LIBSHALOM_SUCCESS;
 }
 
+// LIBSHALOM_SINGLE_PLAN="tile:ksplit" (tile = index into kTileShapes) forces
+// the plan of single GEMMs where it is valid -- a tuning and testing knob,
+// read once.
+void forced_plan(int prec, long M, long N, long K, SinglePlan *p)
+{
+    static const std::pair<int, int> forced = [] {
+        int t = -1, ks = 0;
+        if (const char *e = std::getenv("LIBSHALOM_SINGLE_PLAN"))
+            if (std::sscanf(e, "%d:%d", &t, &ks) != 2) t = -1;
+        return std::make_pair(t, ks);
+    }();
+    const int t = forced.first, ks = forced.second;
+    if (t < 0 || t >= NTILE || ks < 1 || ks > 64 || (prec == P_FP64 && t == T128x128)) return;
+    const int kd = chunk_depth(prec), esize = prec == P_FP64 ? 8 : 4;
+    const long kper = ks == 1 ? std::max<long>(K, 1) : cdiv(cdiv(K, ks), kd) * kd;
+    const int nks = ks == 1 ? 1 : (int)cdiv(K, kper);
+    const TileShape ts = kTileShapes[t];
+    if (nks > 1 && ((size_t)cdiv(M, ts.bm) * cdiv(N, ts.bn) > kCounterBytes / sizeof(int) ||
+                    splitk_slot_bytes(t, M, N, nks, esize) > kSplitKSlotLimit))
+        return;
+    *p = SinglePlan{t, nks, (int)kper};
+}
+
 // One GEMM on `stream`, C = alpha * op(A) op(B) + beta * C with explicit
 // leading dimensions: plan tiles / split-K and launch.
 int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B, long M, long N, long K, long lda,
@@ -229,26 +337,21 @@ int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B,
     d.lda = (int)lda;
     d.ldb = (int)ldb;
     d.ldc = (int)ldc;
-    SinglePlan p = plan_single(prec, M, N, K, mp, info.num_cu);
+    // split-K needs the XCD placement of its slices (k_gemm_tiles); without
+    // it the agent-scope fences make a split slower than none
+    SinglePlan p = plan_single(prec, M, N, K, mp, info.num_cu, kSplitKSlotLimit,
+                               info.xcd_rr ? kCounterBytes / sizeof(int) : 0);
+    if (info.xcd_rr) forced_plan(prec, M, N, K, &p);
     TilesLaunch fn = launch_table().tiles[prec][ta][tb][p.tile];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
-    void *partial = nullptr;
+    int *counters = nullptr;
+    float *slots = nullptr;
     if (p.ksplit > 1) {
-        st = workspace(dev, stream, (size_t)p.ksplit * M * N * esize, &partial);
+        st = fixup_space(dev, stream, splitk_slot_bytes(p.tile, M, N, p.ksplit, esize), &counters, &slots);
         if (st) return st;
     }
-    if (fn(d, p.ksplit, p.kper, partial, alpha, beta, stream) != hipSuccess) return LIBSHALOM_HIP_ERROR;
-    if (p.ksplit > 1) {
-        const long mn = M * N;
-        const int blocks = (int)std::min<long>((mn + 255) / 256, 4L * info.num_cu);
-        if (f64)
-            hipLaunchKernelGGL(k_splitk_reduce<double>, dim3(blocks), dim3(256), 0, stream, (const double *)partial,
-                               (double *)C, M, N, ldc, p.ksplit, alpha, beta);
-        else
-            hipLaunchKernelGGL(k_splitk_reduce<float>, dim3(blocks), dim3(256), 0, stream, (const float *)partial,
-                               (float *)C, M, N, ldc, p.ksplit, (float)alpha, (float)beta);
-        if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
-    }
+    if (fn(d, p.ksplit, p.kper, slots, counters, alpha, beta, info.xcd_rr, stream) != hipSuccess)
+        return LIBSHALOM_HIP_ERROR;
     return LIBSHALOM_SUCCESS;
 }
 
@@ -276,55 +379,6 @@ int valid_precision(int p)
 
 // LIBSHALOM_* precision flag -> kernel precision id (they coincide).
 int kprec(int p) { return p; }
-
-// ---------------------------------------------------------------------------
-// Fused backward: planning of the dA (X) and dB (Y) problems.
-// ---------------------------------------------------------------------------
-// Per (device, stream) memory for the split-reduction fixup: arrival
-// counters (zero between launches; every launch leaves them zero) and the
-// partial-tile slots.
-struct FixupSpace {
-    char *ptr = nullptr;
-    size_t size = 0;
-};
-std::mutex g_fx_mu;
-std::map<std::pair<int, hipStream_t>, FixupSpace> g_fx;
-constexpr size_t kCounterBytes = 1 << 20;  // 256 K tile counters
-
-__global__ void k_zero_ints(int *p, int n)
-{
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
-}
-
-int fixup_space(int dev, hipStream_t s, size_t slot_bytes, int **counters, float **slots)
-{
-    if (const char *f = std::getenv("LIBSHALOM_FAIL_WORKSPACE"))
-        if (f[0] == '1') return LIBSHALOM_ALLOC_FAILED;  // failure injection for tests
-    std::lock_guard<std::mutex> lk(g_fx_mu);
-    FixupSpace &w = g_fx[{dev, s}];
-    const size_t need = kCounterBytes + slot_bytes;
-    if (w.size < need) {
-        if (w.ptr) {
-            if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
-            (void)hipFree(w.ptr);
-            w.ptr = nullptr;
-            w.size = 0;
-        }
-        const size_t sz = std::max(need, (size_t)8 << 20);
-        if (hipMalloc((void **)&w.ptr, sz) != hipSuccess) {
-            (void)hipGetLastError();
-            w.ptr = nullptr;
-            return LIBSHALOM_ALLOC_FAILED;
-        }
-        w.size = sz;
-        // a fill kernel rather than a memset, so that a captured call records a kernel node
-        hipLaunchKernelGGL(k_zero_ints, dim3(256), dim3(256), 0, s, (int *)w.ptr, (int)(kCounterBytes / sizeof(int)));
-        if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
-    }
-    *counters = (int *)w.ptr;
-    *slots = (float *)(w.ptr + kCounterBytes);
-    return LIBSHALOM_SUCCESS;
-}
 
 long ceil_div(long a, long b) { return (a + b - 1) / b; }
 

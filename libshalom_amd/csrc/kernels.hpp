@@ -11,39 +11,147 @@ struct TileShape { int bm, bn; };
 constexpr TileShape kTileShapes[NTILE] = {{32, 32}, {64, 64}, {128, 128}, {128, 32}, {128, 64}, {32, 128}};
 constexpr int NPREC = 5;
 
-// One GEMM (or one K slice of it) over a 1-D grid of tiles.  With ksplit > 1,
-// blockIdx.y is the K slice and the tile goes to partial[slice] (M x N, fp32
-// for the float modes, fp64 for FP64); a reduction kernel sums the slices in
-// slice order, so results do not depend on scheduling.  Without split-K the
-// tile is written as alpha * product + beta * C (C is not read when beta is 0).
-template <class Cfg>
-__global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n, int kper,
-                                                        typename Cfg::T *partial, typename Cfg::T alpha,
-                                                        typename Cfg::T beta)
+// The K slices of one split-K tile summed in slice order into C (the
+// workgroup that arrives last at the tile): C = alpha * sum + beta * C over
+// the tile's tr x tc elements, C not read when beta is 0.  Each thread owns
+// NG groups of 16 bytes of the tile and keeps the loads of a batch of SB
+// slices for all of them in flight before adding (a round trip per slice
+// batch, not per element; eight 16-byte loads in flight keep the kernel's
+// register count where its tile loop puts it).
+template <typename T, int BM, int BN, int NT>
+__device__ __forceinline__ void splitk_sum(const T *__restrict__ slots, int ks, T *__restrict__ C, int ldc, int tr,
+                                           int tc, T alpha, T beta)
 {
+    constexpr int TILE = BM * BN, V = 16 / sizeof(T), NG = TILE / V / NT;
+    static_assert(NG * V * NT == TILE, "whole 16-byte groups per thread");
+    constexpr int SB = NG >= 8 ? 1 : 8 / NG;  // slices per batch
+    typedef T vec __attribute__((ext_vector_type(V)));
+    vec acc[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) acc[g] = *(const vec *)(slots + (size_t)(threadIdx.x + g * NT) * V);
+    for (int s0 = 1; s0 < ks; s0 += SB) {
+        vec t[SB][NG];
+#pragma unroll
+        for (int j = 0; j < SB; ++j)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (s0 + j < ks) t[j][g] = *(const vec *)(slots + (size_t)(s0 + j) * TILE + (threadIdx.x + g * NT) * V);
+#pragma unroll
+        for (int j = 0; j < SB; ++j)
+#pragma unroll
+            for (int g = 0; g < NG; ++g)
+                if (s0 + j < ks) acc[g] += t[j][g];
+    }
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        const int e = (threadIdx.x + g * NT) * V, r = e / BN, c0 = e - r * BN;
+        if (r >= tr) continue;
+        T *o = C + (size_t)r * ldc + c0;
+#pragma unroll
+        for (int i = 0; i < V; ++i)
+            if (c0 + i < tc) o[i] = beta == T(0) ? alpha * acc[g][i] : alpha * acc[g][i] + beta * o[i];
+    }
+}
+
+// The XCD (accelerator complex die) the calling wave runs on: hardware
+// register XCC_ID (id 20), bits 3:0.
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15; }
+
+// Workgroup linear ids of a split-K launch: the dispatcher deals workgroups
+// to the 8 XCDs round-robin (workgroup L to XCD L % 8, verified on the device
+// before such a launch is used), so workgroup L = ((t / 8) * ks + s) * 8 +
+// t % 8 runs slice s of tile t on XCD t % 8: every slice of a tile shares
+// one L2.  Returns the tile, or -1 for the padding past the last tile.
+__device__ __forceinline__ int splitk_tile(int L, int ks, int ntiles, int *slice)
+{
+    const int t = (L / (8 * ks)) * 8 + (L & 7);
+    *slice = (L >> 3) % ks;
+    return t < ntiles ? t : -1;
+}
+
+// After a slice has written its partial tile: publish it and count the
+// arrival at tile t; returns (to every thread) whether this workgroup is the
+// last slice, which then sees every slice's partial tile.  A slice on the
+// tile's XCD waits for its stores (the L1 is write-through, so they are in
+// the XCD's L2, where the other slices read them); one found elsewhere
+// writes its L2 back (agent-scope release) and marks the tile, whose last
+// slice then invalidates its caches (agent-scope acquire) before reading.
+// The counter returns to zero for the next launch.
+__device__ __forceinline__ bool splitk_arrive(int *counters, int t, int ks, bool xcd_local, int *flag)
+{
+    const bool local = xcd_local && xcc_id() == (t & 7);
+    if (local) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_s_waitcnt(0);
+    } else {
+        __threadfence();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int old = atomicAdd(&counters[t], local ? 1 : 0x10001);
+        const bool last = (old & 0xffff) == ks - 1;
+        // 1: last, agent-scope acquire needed; 2: last, on the tile's XCD with every slice
+        *flag = !last ? 0 : (!local || (old >> 16) != 0) ? 1 : 2;
+        if (last) counters[t] = 0;
+    }
+    __syncthreads();
+    const int f = *flag;
+    if (f == 1)
+        __threadfence();
+    else if (f == 2)
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    return f != 0;
+}
+
+// One GEMM over a grid of tiles x K slices.  Without split-K (ks == 1)
+// workgroup b is tile b in row-major order and writes alpha * product + beta
+// * C (C is not read when beta is 0).  With it (the reference accumulates
+// its K blocks in one pass, k_tag in Kernel/SMM_m8n12_new.h, SMM_NN_new.c:
+// 438-459; here the slices run in parallel), the slices of a tile run on one
+// XCD (splitk_tile), every slice writes its partial tile to its slot, and
+// the last slice to arrive (splitk_arrive) sums the slots in slice order --
+// deterministic, in the same launch -- and applies alpha / beta.
+//
+// Tiles in row-major order: the dispatcher deals workgroups to the 8 XCDs
+// round-robin, so the resident tiles of one XCD are every 8th of a band of
+// rows, a compact set whose A and B panels share that XCD's L2.  (Remapping
+// each XCD to a contiguous run of tiles made it sweep whole tile rows, every
+// B panel through one L2, and was up to 5% slower on large grids.)
+template <class Cfg>
+__global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n, int ntiles, int ks, int kper,
+                                                        typename Cfg::T alpha, typename Cfg::T beta,
+                                                        typename Cfg::T *slots, int *counters, int xcd_local)
+{
+    typedef typename Cfg::T T;
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
-    // Tiles in row-major order of blockIdx.x.  The dispatcher deals
-    // workgroups to the 8 XCDs round-robin, so the resident tiles of one XCD
-    // are every 8th of a band of rows: a compact set whose A and B panels
-    // share that XCD's L2.  (Remapping each XCD to a contiguous run of
-    // tiles made it sweep whole tile rows, every B panel through one L2,
-    // and was up to 5% slower on large grids.)
-    const int b = blockIdx.x;
+    __shared__ int flag;
+    int slice = 0;
+    const int b = ks == 1 ? (int)blockIdx.x : splitk_tile(blockIdx.x, ks, ntiles, &slice);
+    if (b < 0) return;
     const int tm = b / tiles_n, tn = b - tm * tiles_n;
     const int m0 = tm * Cfg::BM, n0 = tn * Cfg::BN;
-    const int kbeg = blockIdx.y * kper;
+    const int kbeg = slice * kper;
     const int kend = min(d.K, kbeg + kper);
+    T *slot = slots + ((size_t)b * ks + slice) * (Cfg::BM * Cfg::BN);
     OneSrc src;
-    if (gridDim.y == 1)
-        src.job = tile_job<Cfg>(d, m0, n0, kbeg, kend, d.C, d.ldc);
-    else
-        src.job = tile_job<Cfg>(d, m0, n0, kbeg, kend, partial + (size_t)blockIdx.y * d.M * d.N, d.N);
+    src.job = tile_job<Cfg>(d, m0, n0, kbeg, kend, d.C, d.ldc);
+    if (ks > 1) {  // the partial tile goes to the slice's slot (leading dimension BN)
+        src.job.C = slot;
+        src.job.ldc = Cfg::BN;
+    }
     Tile<Cfg> t;
-    t.scale = gridDim.y == 1 && !(alpha == typename Cfg::T(1) && beta == typename Cfg::T(0));
+    t.scale = ks == 1 && !(alpha == T(1) && beta == T(0));
     t.alpha = alpha;
     t.beta = beta;
     t.stream(src, 0, 1, lds);
+    if (ks == 1 || !splitk_arrive(counters, b, ks, xcd_local != 0, &flag)) return;
+    splitk_sum<T, Cfg::BM, Cfg::BN, Cfg::NT>(slots + (size_t)b * ks * (Cfg::BM * Cfg::BN), ks,
+                                             (T *)d.C + (size_t)m0 * d.ldc + n0, d.ldc, min(Cfg::BM, d.M - m0),
+                                             min(Cfg::BN, d.N - n0), alpha, beta);
 }
+
+// Records the XCD of each of n workgroups (see xcc_id).
+__global__ void k_xcc_probe(int *out, int n);
 
 // Persistent executor: as many workgroups as can be resident stream the job
 // list, workgroup b taking jobs b, b + grid, ...  Two resident workgroups per
@@ -59,29 +167,14 @@ k_gemm_persistent(const GemmDesc *__restrict__ jobs, int njobs)
     t.stream(src, blockIdx.x, gridDim.x, lds);
 }
 
-// C = alpha * (sum over slices of partial[s], in slice order) + beta * C, for
-// the M x N partials and C with leading dimension ldc (C is not read when
-// beta is 0).
-template <typename T>
-__global__ __launch_bounds__(256) void k_splitk_reduce(const T *__restrict__ partial, T *__restrict__ C,
-                                                       long M, long N, long ldc, int nslices, T alpha, T beta)
-{
-    const long mn = M * N;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < mn; i += (long)gridDim.x * blockDim.x) {
-        T s = partial[i];
-        for (int j = 1; j < nslices; ++j) s += partial[(size_t)j * mn + i];
-        const long r = i / N, c = i - r * N;
-        T *o = C + r * ldc + c;
-        *o = beta == T(0) ? alpha * s : alpha * s + beta * *o;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Launcher table (host side).
 // ---------------------------------------------------------------------------
-// alpha / beta are read as the operand type (float or double).
-typedef hipError_t (*TilesLaunch)(const GemmDesc &d, int ksplit, int kper, void *partial, double alpha,
-                                  double beta, hipStream_t s);
+// alpha / beta are read as the operand type (float or double).  With
+// ksplit > 1, slots holds ksplit partial tiles (BM x BN elements of the
+// operand type) per tile and counters one zeroed int per tile.
+typedef hipError_t (*TilesLaunch)(const GemmDesc &d, int ksplit, int kper, void *slots, int *counters,
+                                  double alpha, double beta, bool xcd_local, hipStream_t s);
 typedef hipError_t (*PersistLaunch)(const GemmDesc *jobs, int njobs, int grid, hipStream_t s);
 typedef int (*PersistOccupancy)();  // resident workgroups per CU, 0 on error
 
@@ -97,16 +190,27 @@ struct LaunchTable {
 LaunchTable &launch_table();
 
 template <class Cfg>
-hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *partial, double alpha, double beta,
-                        hipStream_t s)
+hipError_t launch_tiles(const GemmDesc &d, int ksplit, int kper, void *slots, int *counters, double alpha,
+                        double beta, bool xcd_local, hipStream_t s)
 {
     typedef typename Cfg::T T;
     const int tiles_m = (d.M + Cfg::BM - 1) / Cfg::BM, tiles_n = (d.N + Cfg::BN - 1) / Cfg::BN;
     const int ntiles = tiles_m * tiles_n;
-    dim3 grid(ntiles, ksplit);
-    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, grid, dim3(Cfg::NT), 0, s, d, tiles_n, kper, (T *)partial,
-                       (T)alpha, (T)beta);
+    // split-K: tiles padded to a multiple of 8, times the slices (splitk_tile)
+    const long grid = ksplit == 1 ? ntiles : (long)((ntiles + 7) / 8) * 8 * ksplit;
+    hipLaunchKernelGGL(k_gemm_tiles<Cfg>, dim3((unsigned)grid), dim3(Cfg::NT), 0, s, d, tiles_n, ntiles, ksplit,
+                       kper, (T)alpha, (T)beta, (T *)slots, counters, (int)xcd_local);
     return hipGetLastError();
+}
+
+// Bytes of split-K partial-tile slots a plan needs: ksplit tiles of BM x BN
+// elements per tile of C.
+inline size_t splitk_slot_bytes(int tile, long M, long N, int ksplit, int esize)
+{
+    if (ksplit <= 1) return 0;
+    const TileShape ts = kTileShapes[tile];
+    const size_t ntiles = (size_t)((M + ts.bm - 1) / ts.bm) * (size_t)((N + ts.bn - 1) / ts.bn);
+    return ntiles * (size_t)ksplit * ts.bm * ts.bn * esize;
 }
 
 template <class Cfg>

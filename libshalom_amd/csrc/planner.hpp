@@ -10,7 +10,10 @@
 // the CPU tests through LibShalom_query_plan.
 #pragma once
 #include <algorithm>
+#include <cstddef>
 #include <cstdint>
+
+#include "gemm_core.hpp"
 
 namespace ls {
 
@@ -56,47 +59,102 @@ inline double subtile_chunk_cycles(int prec)
 
 inline long cdiv(long a, long b) { return (a + b - 1) / b; }
 
-// Estimated cycles for one GEMM with a given tile and split.
+// The wave grid of each tile shape (TileCfg): nw waves in a gm x gn grid,
+// each owning wm x wn sub-tiles of 16 x 16.
+struct WaveGrid {
+    int nw, gm, gn, wm, wn;
+};
+template <int BM, int BN>
+constexpr WaveGrid wave_grid_of()
+{
+    typedef TileCfg<P_FP32, 0, 0, BM, BN> C;
+    return WaveGrid{C::NW, C::GM, C::GN, C::WM, C::WN};
+}
+inline WaveGrid wave_grid(int tile)
+{
+    static const WaveGrid g[NTILE] = {wave_grid_of<32, 32>(),  wave_grid_of<64, 64>(),  wave_grid_of<128, 128>(),
+                                      wave_grid_of<128, 32>(), wave_grid_of<128, 64>(), wave_grid_of<32, 128>()};
+    return g[tile];
+}
+
+// Waves of a tile that hold part of an M x N GEMM (its first tile).
+inline int live_waves(int tile, long M, long N)
+{
+    const WaveGrid g = wave_grid(tile);
+    const TileShape ts = kTileShapes[tile];
+    const long mt = cdiv(std::min<long>(ts.bm, M), 16), nt = cdiv(std::min<long>(ts.bn, N), 16);
+    int live = 0;
+    for (int w = 0; w < g.nw; ++w) live += (w / g.gn) * g.wm < mt && (w % g.gn) * g.wn < nt;
+    return live;
+}
+
+// MFMA cycles one K chunk of a tile costs on its busiest SIMD (waves w and
+// w + 4 of an eight-wave tile share SIMD w % 4).  FP32 / FP64 waves multiply
+// the sub-tiles that reach into the GEMM; the split modes every sub-tile of
+// a wave holding part of it.
+inline double tile_chunk_mfma(int prec, int tile, long M, long N)
+{
+    const WaveGrid g = wave_grid(tile);
+    const TileShape ts = kTileShapes[tile];
+    const long mt = cdiv(std::min<long>(ts.bm, M), 16), nt = cdiv(std::min<long>(ts.bn, N), 16);
+    const bool split = prec == P_BF16X3 || prec == P_BF16 || prec == P_BF16X6;
+    double simd[4] = {0, 0, 0, 0};
+    for (int w = 0; w < g.nw; ++w) {
+        const long rows = std::max(0L, std::min<long>(g.wm, mt - (w / g.gn) * g.wm));
+        const long cols = std::max(0L, std::min<long>(g.wn, nt - (w % g.gn) * g.wn));
+        simd[w % 4] += rows * cols == 0 ? 0.0 : split ? (double)g.wm * g.wn : (double)rows * cols;
+    }
+    return *std::max_element(simd, simd + 4) * subtile_chunk_cycles(prec);
+}
+
+// Estimated cycles for one GEMM with a given tile and split: rounds of
+// resident workgroups, each a pipeline of K chunks bound by the busier of
+// its MFMA work and its share of the CU's memory bandwidth (co-resident
+// workgroups share both), plus, with K slices, the last slice's sum of the
+// partial tiles.
 inline double plan_cost(int prec, long M, long N, long K, int tile, int ksplit, int num_cu)
 {
     const TileShape ts = kTileShapes[tile];
     const long tm = cdiv(M, ts.bm), tn = cdiv(N, ts.bn);
     const long wgs = tm * tn * ksplit;
     const int occ = tile_occupancy(prec, tile);
-    const long slots = (long)num_cu * occ;
-    const double rounds = (double)cdiv(wgs, slots);
-    // Average rows / cols of C a tile really covers (16-granular).
-    const double rows = std::min<double>(ts.bm, cdiv(M, 16) * 16.0);
-    const double cols = std::min<double>(ts.bn, cdiv(N, 16) * 16.0);
-    const double subtiles_per_wave = std::max(1.0, (rows / 16.0) * (cols / 16.0) / 4.0);
-    const double mfma = subtiles_per_wave * subtile_chunk_cycles(prec) * std::min<double>(occ, 4.0);
+    const double rounds = (double)cdiv(wgs, (long)num_cu * occ);
+    const double co = (double)std::min<long>(occ, cdiv(wgs, num_cu));  // workgroups sharing a CU
+    const double rows = std::min<double>(ts.bm, (double)M), cols = std::min<double>(ts.bn, (double)N);
     const int esize = prec == P_FP64 ? 8 : 4;
-    const double bytes = (rows + cols) * chunk_depth(prec) * esize;
-    // ~8 TB/s over the CUs at 2.4 GHz ~ 13 bytes per cycle per CU, shared by
-    // the co-resident workgroups.
-    const double mem = bytes * std::min<double>((double)occ, (double)cdiv(wgs, num_cu)) / 13.0;
+    const double mfma = tile_chunk_mfma(prec, tile, M, N) * co;
+    // The constants (cycles per workgroup and per chunk, bytes per cycle a
+    // CU's loads sustain, the last slice's cost per partial-tile byte) were
+    // fitted to measured times of every tile and K split on the tall-skinny
+    // and short-wide sgemm_mp grid (M or N in {4096, 16384, 65536}, the other
+    // in {16, 32, 64}, K in {64, 256, 1024}; FP32 and BF16X3): the plans they
+    // choose take 3% longer than the fastest measured plan of each shape.
+    const double mem = (rows + cols) * chunk_depth(prec) * esize * co / 40.0;
     const long kslice = cdiv(K, ksplit);
     const double chunks = (double)std::max<long>(1, cdiv(kslice, chunk_depth(prec)));
-    const double per_wg = 1500.0 + chunks * (std::max(mfma, mem) + 150.0);
-    double total = rounds * per_wg;
-    if (ksplit > 1) total += 3000.0 + (double)ksplit * M * N * esize / (13.0 * num_cu);
+    double total = rounds * (12000.0 + chunks * (std::max(mfma, mem) + 150.0));
+    if (ksplit > 1)  // partial tile written, then read back by the last slice
+        total += 2500.0 + rows * cols * esize * (1.0 + ksplit) / 40.0;
     return total;
 }
 
-inline SinglePlan plan_single(int prec, long M, long N, long K, bool /*mp*/, int num_cu)
+// The plan of one GEMM: the tile shape and power-of-two K split of least
+// estimated cost, with at most max_slot_bytes of partial tiles and
+// max_split_tiles tiles when K is split (0: no split).
+inline SinglePlan plan_single(int prec, long M, long N, long K, bool /*mp*/, int num_cu,
+                              size_t max_slot_bytes = (size_t)256 << 20, size_t max_split_tiles = 1 << 18)
 {
     SinglePlan best{T32x32, 1, (int)std::max<long>(K, 1)};
     double best_cost = 1e300;
     const int kd = chunk_depth(prec);
+    const int esize = prec == P_FP64 ? 8 : 4;
     for (int t = 0; t < NTILE; ++t) {
         if (prec == P_FP64 && t == T128x128) continue;  // 128 accumulator registers: occupancy 1
-        // The 128 x 128 tile's eight waves own fixed 32 x 64 blocks: a GEMM
-        // with a dimension of at most 64 leaves half of them idle, which the
-        // FP32 MFMAs (the slowest per element) cannot afford; the tall and
-        // wide tiles split the rows (columns) over all their waves instead.
-        if (prec == P_FP32 && t == T128x128 && std::min(M, N) <= 64) continue;
+        const TileShape ts = kTileShapes[t];
+        const size_t ntiles = (size_t)cdiv(M, ts.bm) * (size_t)cdiv(N, ts.bn);
         for (int ks = 1; ks <= 64; ks *= 2) {
             if (ks > 1 && K < (long)ks * 4 * kd) break;  // keep >= 4 chunks per slice
+            if (ks > 1 && (ntiles > max_split_tiles || ntiles * ks * ts.bm * ts.bn * esize > max_slot_bytes)) break;
             const double c = plan_cost(prec, M, N, K, t, ks, num_cu);
             if (c < best_cost * 0.999) {
                 best_cost = c;

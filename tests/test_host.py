@@ -225,6 +225,37 @@ def test_planner_fills_the_chip():
     assert p["ksplit"] == 1
 
 
+# The wave grid of each tile (TileCfg in gemm_core.hpp): 128 x 128 tiles run
+# eight waves in a 4 x 2 grid of 32 x 64 blocks; the other square tiles four
+# waves in a 2 x 2 grid; the tall tiles (128 x 32, 128 x 64) split their rows
+# over four waves, the wide one (32 x 128) its columns.
+WAVE_BLOCK = {(32, 32): (16, 16), (64, 64): (32, 32), (128, 128): (32, 64), (128, 32): (32, 32),
+              (128, 64): (32, 64), (32, 128): (32, 32)}
+
+
+def idle_fraction(tile, M, N):
+    """Fraction of a tile's waves holding no part of an M x N GEMM (its first tile)."""
+    bm, bn = tile
+    wr, wc = WAVE_BLOCK[tile]
+    waves_r, waves_c = bm // wr, bn // wc
+    live_r = min(waves_r, -(-min(bm, M) // wr))
+    live_c = min(waves_c, -(-min(bn, N) // wc))
+    return 1.0 - live_r * live_c / (waves_r * waves_c)
+
+
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3, ls.BF16X6, ls.FP64])
+def test_planner_keeps_waves_busy_on_skinny_grid(prec):
+    """On the tall-skinny / short-wide sgemm_mp grid, the chosen tile never
+    leaves more than half its waves idle when another tile would not."""
+    from libshalom_amd.workloads import TALL_SKINNY
+    shapes = list(TALL_SKINNY) + [(N, M, K) for (M, N, K) in TALL_SKINNY]
+    for (M, N, K) in shapes:
+        tile = ls.query_plan(prec, M, N, K, num_cu=256, mp=True)["tile"]
+        if idle_fraction(tile, M, N) > 0.5:
+            others = [t for t in WAVE_BLOCK if not (prec == ls.FP64 and t == (128, 128))]
+            assert all(idle_fraction(t, M, N) > 0.5 for t in others), (M, N, K, tile)
+
+
 def test_planner_errors():
     with pytest.raises(ls.LibShalomError):
         ls.query_plan(7, 4, 4, 4)
