@@ -118,6 +118,21 @@ int LibShalom_dgemm_general(int transa, int transb, long M, long N, long K, doub
                             long lda, const double *B, long ldb, double beta, double *C, long ldc,
                             LibShalom_stream_t stream);
 
+/* LibShalom_sgemm_general, allocating nothing (no counterpart in the
+ * reference): for callers that cannot allocate device memory -- code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments, results and errors.  Its split-K scratch (arrival counters
+ * and partial tiles) is a static device array of 64 KiB + 32 MiB per GPU,
+ * held until the process ends; the planner picks fewer K slices (or none)
+ * where more would not fit, and a plan needing more returns
+ * LIBSHALOM_NOT_SUPPORTED.  The first call on a device takes the array's
+ * address and zeroes its counters (a fill kernel and a wait on the null
+ * stream: make that first call outside a stream capture).  Calls that
+ * overlap on one GPU share the array and race: serialize them. */
+int LibShalom_sgemm_general_noalloc(int transa, int transb, long M, long N, long K, float alpha,
+                                    const float *A, long lda, const float *B, long ldb, float beta, float *C,
+                                    long ldc, int precision, LibShalom_stream_t stream);
+
 /* Batched GEMMs of uneven sizes (no reference counterpart; the reference
  * makes one call per GEMM, e.g. benchmark/CP2K_kernels/LibShalom_dgemm.c:102-106).
  * A, B, C, M, N, K are HOST arrays of `count` entries (device pointers and

@@ -19,7 +19,7 @@ import os
 __all__ = [
     "NoTrans", "Trans", "FP32", "BF16X3", "BF16", "BF16X6", "FP64",
     "LibShalomError", "lib", "sgemm", "sgemm_mp", "dgemm", "sgemm_ex", "sgemm_mp_ex",
-    "dgemm_ex", "sgemm_general", "dgemm_general", "set_thread_nums", "set_thread_ways", "get_thread_config",
+    "dgemm_ex", "sgemm_general", "sgemm_general_noalloc", "dgemm_general", "set_thread_nums", "set_thread_ways", "get_thread_config",
     "set_precision", "get_precision", "BatchedPlan", "sgemm_batched", "dgemm_batched",
     "sgemm_backward", "sgemm_backward_strided", "sgemm_strided_batched", "autotune_batched",
     "autotune_reset", "query_tuned_tile", "query_plan", "query_batched_tile", "LIB_PATH",
@@ -75,6 +75,8 @@ SIGNATURES = {
     "LibShalom_dgemm_ex": (_I, [_I, _I, _P, _P, _P, _L, _L, _L, _P]),
     "LibShalom_sgemm_general": (_I, [_I, _I, _L, _L, _L, ct.c_float, _P, _L, _P, _L, ct.c_float, _P, _L, _I, _P]),
     "LibShalom_dgemm_general": (_I, [_I, _I, _L, _L, _L, ct.c_double, _P, _L, _P, _L, ct.c_double, _P, _L, _P]),
+    "LibShalom_sgemm_general_noalloc": (_I, [_I, _I, _L, _L, _L, ct.c_float, _P, _L, _P, _L, ct.c_float, _P, _L, _I,
+                                             _P]),
     "LibShalom_sgemm_batched_plan": (_I, [ct.POINTER(_P), _I, _I, _P, _P, _P, _P, _P, _P, _L, _I]),
     "LibShalom_dgemm_batched_plan": (_I, [ct.POINTER(_P), _I, _I, _P, _P, _P, _P, _P, _P, _L]),
     "LibShalom_plan_execute": (_I, [_P, _P]),
@@ -235,6 +237,15 @@ def sgemm_general(transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 
     (BLAS-style; no reference counterpart).  C is not read when beta is 0."""
     _check(_lib.LibShalom_sgemm_general(transa, transb, M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb, beta,
                                         _ptr(C), ldc, precision, _stream(stream)), "LibShalom_sgemm_general")
+
+
+def sgemm_general_noalloc(transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, precision=FP32,
+                          stream=None):
+    """sgemm_general without any device allocation: its split-K scratch is a
+    static device array per GPU (calls that overlap on one GPU race)."""
+    _check(_lib.LibShalom_sgemm_general_noalloc(transa, transb, M, N, K, alpha, _ptr(A), lda, _ptr(B), ldb, beta,
+                                                _ptr(C), ldc, precision, _stream(stream)),
+           "LibShalom_sgemm_general_noalloc")
 
 
 def dgemm_general(transa, transb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, stream=None):

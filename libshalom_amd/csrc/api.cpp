@@ -323,8 +323,51 @@ void forced_plan(int prec, long M, long N, long K, SinglePlan *p)
 
 // One GEMM on `stream`, C = alpha * op(A) op(B) + beta * C with explicit
 // leading dimensions: plan tiles / split-K and launch.
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (LibShalom_sgemm_general_noalloc):
+// split-K arrival counters and partial-tile slots in a file-scope device
+// array, one copy per GPU, held until the process ends.  Host code uses only
+// the device address hipGetSymbolAddress returns for the current device
+// (noalloc_scratch), never the array's name.  The counters are zeroed once,
+// when that address is taken, and every launch leaves them zero again.
+// Calls that overlap on one GPU share the array and race.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocCounterBytes = (size_t)64 << 10;  // 16 K tile counters
+constexpr size_t kNoAllocSlotBytes = (size_t)32 << 20;     // partial tiles
+__device__ __align__(256) char scratch[kNoAllocCounterBytes + kNoAllocSlotBytes];
+std::mutex g_noalloc_mu;
+std::map<int, char *> g_noalloc_ptr;
+
+int noalloc_scratch(int dev, char **out)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    auto it = g_noalloc_ptr.find(dev);
+    if (it != g_noalloc_ptr.end()) {
+        *out = it->second;
+        return LIBSHALOM_SUCCESS;
+    }
+    char *ptr = nullptr;
+    if (hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        return LIBSHALOM_HIP_ERROR;
+    }
+    // a fill kernel, then wait: the counters must be zero before any use
+    hipLaunchKernelGGL(k_zero_ints, dim3(64), dim3(256), 0, nullptr, (int *)ptr,
+                       (int)(kNoAllocCounterBytes / sizeof(int)));
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+        (void)hipGetLastError();
+        return LIBSHALOM_HIP_ERROR;
+    }
+    g_noalloc_ptr[dev] = ptr;
+    *out = ptr;
+    return LIBSHALOM_SUCCESS;
+}
+
+// One GEMM on `stream`: plan tiles / split-K and launch.  With noalloc the
+// split-K counters and slots come from the static scratch array and the plan
+// is limited to what it holds; otherwise from the per-stream fixup space.
 int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B, long M, long N, long K, long lda,
-                long ldb, long ldc, double alpha, double beta, bool mp, hipStream_t stream)
+                long ldb, long ldc, double alpha, double beta, bool mp, hipStream_t stream, bool noalloc = false)
 {
     int dev;
     DeviceInfo info;
@@ -339,16 +382,27 @@ int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B,
     d.ldc = (int)ldc;
     // split-K needs the XCD placement of its slices (k_gemm_tiles); without
     // it the agent-scope fences make a split slower than none
-    SinglePlan p = plan_single(prec, M, N, K, mp, info.num_cu, kSplitKSlotLimit,
-                               info.xcd_rr ? kCounterBytes / sizeof(int) : 0);
-    if (info.xcd_rr) forced_plan(prec, M, N, K, &p);
+    const size_t slot_limit = noalloc ? kNoAllocSlotBytes : kSplitKSlotLimit;
+    const size_t counter_limit = (noalloc ? kNoAllocCounterBytes : kCounterBytes) / sizeof(int);
+    SinglePlan p = plan_single(prec, M, N, K, mp, info.num_cu, slot_limit, info.xcd_rr ? counter_limit : 0);
+    if (info.xcd_rr && !noalloc) forced_plan(prec, M, N, K, &p);
     TilesLaunch fn = launch_table().tiles[prec][ta][tb][p.tile];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
     int *counters = nullptr;
     float *slots = nullptr;
     if (p.ksplit > 1) {
-        st = fixup_space(dev, stream, splitk_slot_bytes(p.tile, M, N, p.ksplit, esize), &counters, &slots);
-        if (st) return st;
+        const size_t need = splitk_slot_bytes(p.tile, M, N, p.ksplit, esize);
+        if (noalloc) {
+            char *ptr;
+            if (need > kNoAllocSlotBytes) return LIBSHALOM_NOT_SUPPORTED;
+            st = noalloc_scratch(dev, &ptr);
+            if (st) return st;
+            counters = (int *)ptr;
+            slots = (float *)(ptr + kNoAllocCounterBytes);
+        } else {
+            st = fixup_space(dev, stream, need, &counters, &slots);
+            if (st) return st;
+        }
     }
     if (fn(d, p.ksplit, p.kper, slots, counters, alpha, beta, info.xcd_rr, stream) != hipSuccess)
         return LIBSHALOM_HIP_ERROR;
@@ -1417,6 +1471,20 @@ int LibShalom_sgemm_general(int transa, int transb, long M, long N, long K, floa
     if (!st)
         st = run_general(kprec(precision), transa, transb, C, A, B, M, N, K, lda, ldb, ldc, alpha, beta, false,
                          (hipStream_t)stream);
+    return set_status(st);
+}
+
+int LibShalom_sgemm_general_noalloc(int transa, int transb, long M, long N, long K, float alpha, const float *A,
+                                    long lda, const float *B, long ldb, float beta, float *C, long ldc, int precision,
+                                    LibShalom_stream_t stream)
+{
+    int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
+    if (!st) st = check_dims(transa, transb, M, N, K);
+    if (!st) st = check_lds(transa, transb, M, N, K, lda, ldb, ldc);
+    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    if (!st)
+        st = run_general(kprec(precision), transa, transb, C, A, B, M, N, K, lda, ldb, ldc, alpha, beta, false,
+                         (hipStream_t)stream, true);
     return set_status(st);
 }
 
