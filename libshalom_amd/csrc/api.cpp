@@ -469,7 +469,7 @@ void bwd_slice(BwdProblem &P, int T, long len)
     const long outs = shared ? 1 : P.nimg;
     const long total = shared ? (long)P.nimg * P.red : P.red;
     if (P.ntiles == 0) {
-        P.units = 0;
+        P.units = P.grid = 0;
         P.nslices = 1;
         return;
     }
@@ -494,6 +494,8 @@ void bwd_slice(BwdProblem &P, int T, long len)
         P.nslices = (int)ceil_div(P.nimg, g);
     }
     P.units = (int)(outs * P.ntiles * P.nslices);
+    // with slices, the tile groups padded to a multiple of 8 (unit_of)
+    P.grid = P.nslices > 1 ? (int)(((outs * P.ntiles + 7) / 8) * 8 * P.nslices) : P.units;
 }
 
 // Tile size and reduction slices for problems that share one launch.
@@ -622,6 +624,7 @@ int backward_run(int prec, int ta, int tb, const float *dC, const float *A, cons
     if (T <= 0) return -T;
     BwdLaunch fn = bwd_table().fn[prec][ta][tb][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    X.xcd = Y.xcd = info.xcd_rr;
     return fn(X, Y, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }
 
@@ -640,6 +643,7 @@ int strided_run(int prec, int ta, int tb, float *C, const float *A, const float 
     if (T <= 0) return -T;
     StridedLaunch fn = bwd_table().strided[prec][ta][tb][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    P.xcd = info.xcd_rr;
     return fn(P, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }
 
@@ -774,6 +778,7 @@ int conv_bwd_run(int prec, float *dX, float *dW, const float *dY, const float *W
     const bool runs = sw == 1 && Q % 4 == 0;
     ConvBwdLaunch fn = bwd_table().conv_bwd[prec][T == 128][runs];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    X_.xcd = Y_.xcd = info.xcd_rr;
     return fn(X_, Y_, gx, gy, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }
 
@@ -800,6 +805,7 @@ int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch
     if (T <= 0) return -T;
     ConvLaunch fn = bwd_table().conv_fwd[prec][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
+    Pb.xcd = info.xcd_rr;
     return fn(Pb, g, s) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
 }
 

@@ -36,7 +36,10 @@ struct BwdProblem {
     int slen;             // spi > 0: reduction length of a slice (multiple of kalign)
     int kalign;           // slice lengths are multiples of this (the K chunk, and a
                           // convolution's R*S so that a slice starts on a channel)
-    int units;            // workgroups of this problem
+    int units;            // units of work of this problem
+    int grid;             // workgroups of this problem: units, and with slices the
+                          // tiles padded to a multiple of 8 (unit_of)
+    int xcd;              // slices of a tile share an XCD (see splitk_arrive)
     float *slots;         // nslices > 1: partial tiles, [tile][slice][BM * BN]
     int *counters;        // nslices > 1: arrivals per tile (zero between launches)
     // A convolution output with the batch folded into its columns (fold_total
@@ -100,19 +103,12 @@ struct SliceSrc {
     }
 };
 
-// Sum the slots of tile tg in slice order into the output (last arrival).
+// Sum the slots of tile tg in slice order into the output (the last slice
+// to arrive, see splitk_arrive).
 template <int BM, int BN, int NT>
 __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_out, int r0, int c0, int *flag)
 {
-    __threadfence();  // this workgroup's partial tile is visible device-wide
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int old = atomicAdd(&P.counters[tg], 1);
-        *flag = old == P.nslices - 1;
-    }
-    __syncthreads();
-    if (!*flag) return;
-    __threadfence();  // and the other slices' tiles are visible here
+    if (!splitk_arrive(P.counters, tg, P.nslices, P.xcd != 0, flag)) return;
     const int tr = min(BM, P.R - r0), tcn = min(BN, P.Cc - c0);
     const float *base = P.slots + (size_t)tg * P.nslices * (BM * BN);
     float *O = (float *)P.O + (long long)img_out * P.sO;
@@ -140,14 +136,30 @@ __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_o
             o[col] = acc[q];
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) P.counters[tg] = 0;  // ready for the next launch
+}
+
+// Unit of workgroup u of a problem: tile group tg (output tile, and image
+// for unshared outputs) and slice s.  With slices, workgroup u runs slice
+// (u / 8) % nslices of tile group (u / (8 * nslices)) * 8 + u % 8, so the
+// slices of a tile share XCD tg % 8 (workgroup u runs on XCD u % 8; the
+// launch starts every problem at a multiple of 8).  tg < 0 marks the
+// padding past the last tile group.
+struct Unit {
+    int tg, s;  // tg < 0: padding
+};
+__device__ __forceinline__ Unit unit_of(int nslices, int units, int u)
+{
+    if (nslices == 1) return Unit{u < units ? u : -1, 0};
+    const int tg = (u / (8 * nslices)) * 8 + (u & 7);
+    return Unit{tg < units / nslices ? tg : -1, (u >> 3) % nslices};
 }
 
 template <class Cfg, int IM2COL = kPlainB>
 __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, int *flag, const ConvGeom &geom = {})
 {
-    const int tg = u / P.nslices, s = u - tg * P.nslices;
+    const Unit un = unit_of(P.nslices, P.units, u);
+    if (un.tg < 0) return;
+    const int tg = un.tg, s = un.s;
     const bool shared = P.sO == 0;
     const int img_out = shared ? 0 : tg / P.ntiles;
     const int t = shared ? tg : tg - img_out * P.ntiles;
@@ -214,18 +226,19 @@ struct BwdLds {
     static constexpr int BYTES = CX::LDS_BYTES > CY::LDS_BYTES ? CX::LDS_BYTES : CY::LDS_BYTES;
 };
 
-// Units [0, X.units) are dA units, the rest dB units.
+// Workgroups [0, X.grid rounded up to a multiple of 8) are dA units, the rest
+// dB units.
 template <class CX, class CY>
 __global__ __attribute__((amdgpu_flat_work_group_size(CX::NT, CX::NT))) void
 k_gemm_backward(BwdProblem X, BwdProblem Y)
 {
     __shared__ __attribute__((aligned(16))) char lds[BwdLds<CX, CY>::BYTES];
     __shared__ int flag;
-    const int u = blockIdx.x;
-    if (u < X.units)
+    const int u = blockIdx.x, xg = (X.grid + 7) & ~7;  // Y starts on a multiple of 8
+    if (u < xg)
         bwd_unit<CX>(X, u, lds, &flag);
     else
-        bwd_unit<CY>(Y, u - X.units, lds, &flag);
+        bwd_unit<CY>(Y, u - xg, lds, &flag);
 }
 
 // One problem on its own: a strided batch of GEMMs C_i = op(A_i) op(B_i)
@@ -266,11 +279,11 @@ k_conv_backward(BwdProblem X, BwdProblem Y, ConvGeom gx, ConvGeom gy)
 {
     __shared__ __attribute__((aligned(16))) char lds[BwdLds<CX, CY>::BYTES];
     __shared__ int flag;
-    const int u = blockIdx.x;
-    if (u < X.units)
+    const int u = blockIdx.x, xg = (X.grid + 7) & ~7;  // Y starts on a multiple of 8
+    if (u < xg)
         bwd_unit<CX, kIm2col>(X, u, lds, &flag, gx);
     else
-        bwd_unit<CY, IMY>(Y, u - X.units, lds, &flag, gy);
+        bwd_unit<CY, IMY>(Y, u - xg, lds, &flag, gy);
 }
 
 // ---------------------------------------------------------------------------
@@ -290,8 +303,8 @@ hipError_t launch_conv_backward(const BwdProblem &X, const BwdProblem &Y, const 
     typedef TileCfg<PREC, 0, 0, BT, BT> CX;  // dX: flipped filter x im2col(dY)
     typedef TileCfg<PREC, 0, 1, BT, BT> CY;  // dW: dY x im2col(X)^T
     static_assert(CX::NT == CY::NT, "one workgroup size for both problems");
-    const int grid = X.units + Y.units;
-    if (grid <= 0) return hipSuccess;
+    if (X.units + Y.units <= 0) return hipSuccess;
+    const int grid = ((X.grid + 7) & ~7) + Y.grid;
     hipLaunchKernelGGL((k_conv_backward<CX, CY, RUNS ? kIm2colTR : kIm2colT>), dim3(grid), dim3(CX::NT), 0, s, X, Y,
                        gx, gy);
     return hipGetLastError();
@@ -302,7 +315,7 @@ hipError_t launch_conv_fwd(const BwdProblem &P, const ConvGeom &g, hipStream_t s
 {
     typedef TileCfg<PREC, 0, 0, BT, BT> Cfg;
     if (P.units <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_conv_fwd<Cfg>, dim3(P.units), dim3(Cfg::NT), 0, s, P, g);
+    hipLaunchKernelGGL(k_conv_fwd<Cfg>, dim3(P.grid), dim3(Cfg::NT), 0, s, P, g);
     return hipGetLastError();
 }
 
@@ -311,7 +324,7 @@ hipError_t launch_strided(const BwdProblem &P, hipStream_t s)
 {
     typedef TileCfg<PREC, TA, TB, BT, BT> Cfg;
     if (P.units <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gemm_strided<Cfg>, dim3(P.units), dim3(Cfg::NT), 0, s, P);
+    hipLaunchKernelGGL(k_gemm_strided<Cfg>, dim3(P.grid), dim3(Cfg::NT), 0, s, P);
     return hipGetLastError();
 }
 
@@ -325,8 +338,8 @@ hipError_t launch_backward(const BwdProblem &X, const BwdProblem &Y, hipStream_t
 {
     typedef TileCfg<PREC, kBwdTA[TA][TB][0], kBwdTB[TA][TB][0], BT, BT> CX;
     typedef TileCfg<PREC, kBwdTA[TA][TB][1], kBwdTB[TA][TB][1], BT, BT> CY;
-    const int grid = X.units + Y.units;
-    if (grid <= 0) return hipSuccess;
+    if (X.units + Y.units <= 0) return hipSuccess;
+    const int grid = ((X.grid + 7) & ~7) + Y.grid;
     static_assert(CX::NT == CY::NT, "one workgroup size for both problems");
     hipLaunchKernelGGL((k_gemm_backward<CX, CY>), dim3(grid), dim3(CX::NT), 0, s, X, Y);
     return hipGetLastError();
