@@ -12,12 +12,14 @@ elements outside |C - C64| <= 2 K 2^-24 (|A||B|).  The headline value is the
 fastest mode with no such element: the metric is quoted at FP32-level error,
 which the one- and two-piece splits (BF16, BF16X3) cannot meet.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+The same line carries, under "configs", the metric's other configurations
+run in the same call (compact_configs): the 80^3 NT call, the cubic 8..128
+sweep, tall-skinny / short-wide sgemm_mp, 100k batched CP2K-sized DGEMMs and
+the im2col convolution layers forward + fused backward (batch 32).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-configs]
        python bench.py --workload {conv,mp,sweep,hello,dgemm_batched}
-The other workloads print one JSON line each of their own (the batched SGEMM
-line above is the headline): the im2col convolution GEMMs forward + fused
-backward (batch 32), tall-skinny / short-wide sgemm_mp, the cubic 8..128
-sweep, the 80^3 NT call, and 100k batched CP2K-sized DGEMMs.
+A --workload run prints that configuration's full results as its own line.
 """
 import argparse
 import json
@@ -51,6 +53,9 @@ def main():
                     help="GEMMs drawn at random for the error check (plus 8 per K class 1..16)")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="batched plans with the fixed tile rule instead of the autotuned tiles")
+    ap.add_argument("--no-configs", dest="configs", action="store_false",
+                    help="skip the other configurations (Hello 80^3, the cubic sweep, sgemm_mp, the DGEMM batch, "
+                         "the conv layers) the default run adds to its JSON line")
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "conv", "mp", "sweep", "hello", "dgemm_batched"])
     ap.add_argument("--conv-layers", default="",
@@ -72,7 +77,7 @@ def main():
 
     dev = torch.device("cuda:0")
     if args.workload != "batched":
-        OTHER[args.workload](args, dev)
+        emit(OTHER[args.workload](args, dev))
         return
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib * (1 << 20) // 4
@@ -184,6 +189,12 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline(shapes, args.cpu_seconds)
         log(f"cpu baseline: {cpu}")
+    sample = {"gemms": checker.count, "elements": checker.elements, "k_le_16_classes": checker.small_k_classes}
+    configs = None
+    if args.configs:
+        del dA, dB, dC, checker
+        torch.cuda.empty_cache()
+        configs = compact_configs(args, dev)
 
     out = {
         "metric": "batched small-GEMM GFLOP/s (bf16x3 vs FP32 MFMA) at FP32-level error; % roofline",
@@ -203,8 +214,7 @@ def main():
                    "gb_per_step": round(nbytes / 1e9, 4)},
         "value_mode": vmode,
         "value_rule": "fastest mode with 0 elements outside |C-C64| <= 2K 2^-24 (|A||B|) on the error sample",
-        "error_sample": {"gemms": checker.count, "elements": checker.elements,
-                         "k_le_16_classes": checker.small_k_classes},
+        "error_sample": sample,
         "speedup_bf16x3_vs_fp32": (round(results["bf16x3"]["gflops"] / results["fp32"]["gflops"], 3)
                                    if "fp32" in results and "bf16x3" in results else None),
         "modes": {k: {kk: (float(f"{vv:.4g}") if isinstance(vv, float) else vv) for kk, vv in v.items()}
@@ -216,6 +226,7 @@ def main():
                      "kernel_ms": round(kms, 4), "kernel_algorithmic_bytes": kbytes,
                      "kernel_tflops": round(kflops / (kms * 1e-3) / 1e12, 2)},
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     print(json.dumps(out), flush=True)
 
@@ -504,7 +515,7 @@ def bench_conv(args, dev):
         layers.append(row)
         del W, X, dY, Y, dW, dX
         torch.cuda.empty_cache()
-    emit({"metric": "im2col conv GEMM forward + fused backward GFLOP/s, batch 32 (four layers)",
+    return ({"metric": "im2col conv GEMM forward + fused backward GFLOP/s, batch 32 (four layers)",
           "value": round(tot_f["bf16x3"] / tot_ms["bf16x3"] / 1e6, 1), "unit": "GFLOP/s",
           "fp32_gflops": round(tot_f["fp32"] / tot_ms["fp32"] / 1e6, 1), "n_gpus": 1,
           "dtype": "bf16x3 (value) and fp32", "data": "synthetic: W~N(0,2/K), X,dC~N(0,1)",
@@ -556,7 +567,7 @@ def bench_mp(args, dev):
     tf = sum(2.0 * r["M"] * r["N"] * r["K"] for r in rows)
     t32 = sum(r["fp32"]["ms"] for r in rows)
     t3 = sum(r["bf16x3"]["ms"] for r in rows)
-    emit({"metric": "sgemm_mp tall-skinny / short-wide GFLOP/s (all shapes, NN+NT)",
+    return ({"metric": "sgemm_mp tall-skinny / short-wide GFLOP/s (all shapes, NN+NT)",
           "value": round(tf / t3 / 1e6, 1), "unit": "GFLOP/s", "fp32_gflops": round(tf / t32 / 1e6, 1),
           "n_gpus": 1, "dtype": "bf16x3 (value) and fp32", "data": "synthetic: U[-1,1)",
           "config": {"workload": "M in {4096,16384,65536} x N in {16,32,64} x K in {64,256,1024} and "
@@ -580,9 +591,14 @@ def bench_sweep(args, dev):
         row = {"n": n}
         for ta in (0, 1):
             for tb in (0, 1):
+                C.fill_(float("nan"))
                 ms = time_ms(lambda: ls.sgemm_ex(ta, tb, C, A, B, n, n, n, ls.FP32, stream), 100, 5, stream)
+                a = A.double().T if ta else A.double()
+                b = B.double().T if tb else B.double()
+                _, _, e, den = rel_err(C, a @ b, a.abs() @ b.abs())
+                viol = int((e > 2 * n * 2.0 ** -24 * den).sum()) + int(torch.isnan(C).sum())
                 row[f"{'NT'[ta]}{'NT'[tb]}"] = {"us": round(ms * 1e3, 2),
-                                                "gflops": round(2.0 * n ** 3 / ms / 1e6, 2)}
+                                                "gflops": round(2.0 * n ** 3 / ms / 1e6, 2), "viol": viol}
         rows.append(row)
         log(f"sweep n={n}: " + " ".join(f"{k} {v['us']}us" for k, v in row.items() if k != "n"))
     cpu = None
@@ -599,7 +615,7 @@ def bench_sweep(args, dev):
             el = (time.perf_counter() - t0) / 100
             cpu.append({"n": n, "us": round(el * 1e6, 2), "gflops": round(2.0 * n ** 3 / el / 1e9, 2)})
         cpu = {"kind": "port", "cores": oracle.cpu_threads(), "sizes": cpu}
-    emit({"metric": "small SGEMM sweep: per-call time, n=8..128 step 8, FP32", "value": rows[-1]["NN"]["gflops"],
+    return ({"metric": "small SGEMM sweep: per-call time, n=8..128 step 8, FP32", "value": rows[-1]["NN"]["gflops"],
           "unit": "GFLOP/s at n=128 NN", "n_gpus": 1, "dtype": "fp32", "data": "synthetic: U[-1,1)",
           "config": {"workload": "cubic n=8..128 step 8 x {NN,NT,TN,TT}, 5 warm-ups + 100 calls"},
           "sizes": rows, "cpu_baseline": cpu})
@@ -621,12 +637,21 @@ def bench_hello(args, dev):
     for _ in range(100):
         ls.sgemm(0, 1, C, At, Bt, 80, 80, 80)
     sync_us = (time.perf_counter() - t0) / 100 * 1e6
+    # the reference's own ABI: host buffers (staged through the device)
+    import numpy as np
+    Ch = np.empty((80, 80), np.float32)
+    ls.sgemm(0, 1, Ch, A, B, 80, 80, 80)
+    t0 = time.perf_counter()
+    for _ in range(100):
+        ls.sgemm(0, 1, Ch, A, B, 80, 80, 80)
+    host_us = (time.perf_counter() - t0) / 100 * 1e6
     a = At.double().view(80, 80)
     b = Bt.double().view(80, 80).T
     mx, mean, e, den = rel_err(C, a @ b, a.abs() @ b.abs())
     viol = int((e > 2 * 80 * 2.0 ** -24 * den).sum())
-    emit({"metric": "Hello 80x80x80 NT sgemm", "value": round(ms * 1e3, 2), "unit": "us per call (stream)",
-          "higher_is_better": False, "sync_api_us": round(sync_us, 2), "gflops": round(1.024e6 / ms / 1e6, 2),
+    return ({"metric": "Hello 80x80x80 NT sgemm", "value": round(ms * 1e3, 2), "unit": "us per call (stream)",
+          "higher_is_better": False, "sync_api_us": round(sync_us, 2), "host_pointer_api_us": round(host_us, 2),
+          "gflops": round(1.024e6 / ms / 1e6, 2),
           "violations": viol, "max_rel": float(f"{mx:.3g}"), "n_gpus": 1, "dtype": "fp32",
           "data": "drand48 default-state inputs (reference harness recipe)",
           "config": {"workload": "LibShalom_sgemm(NoTrans, Trans) M=N=K=80"}})
@@ -685,7 +710,7 @@ def bench_dgemm_batched(args, dev):
         sf = float(np.sum(2.0 * Ms[sel] * Ns[sel] * Ks[sel]))
         cpu = {"value": round(sf * reps / el / 1e9, 2), "unit": "GFLOP/s", "cores": oracle.cpu_threads(),
                "kind": "port", "sample": f"first 2000 GEMMs x {reps} reps, FP64 OpenMP triple loop"}
-    emit({"metric": "batched DGEMM GFLOP/s, 100k GEMMs with CP2K block dims", "value": round(flops / ms / 1e6, 1),
+    return ({"metric": "batched DGEMM GFLOP/s, 100k GEMMs with CP2K block dims", "value": round(flops / ms / 1e6, 1),
           "unit": "GFLOP/s", "ms_per_step": round(ms, 4), "hbm_gbs": round(gbs, 1),
           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                        "frac": round(gbs / PEAK_HBM_GBS, 4)},
@@ -693,6 +718,64 @@ def bench_dgemm_batched(args, dev):
           "config": {"workload": "100k DGEMMs, M,N,K from {4,5,9,13,16,22,23,26,32}, NN",
                      "gflop_per_step": round(flops / 1e9, 4), "gb_per_step": round(nbytes / 1e9, 4)},
           "cpu_baseline": cpu})
+
+
+def r3(x):
+    return float(f"{x:.3g}")
+
+
+def compact_configs(args, dev):
+    """Every configuration of the metric's benchmark set, run in the default
+    bench.py call and summarised compactly for its one JSON line: cfg1 the
+    80^3 NT call, cfg2 the cubic sweep (microseconds per call, four trans
+    combinations), cfg3 the tall-skinny / short-wide sgemm_mp grid (per shape:
+    M, N, K, transb, then for FP32 and BF16X3 microseconds, fraction of
+    min(MFMA peak, AI x HBM peak) and max relative error), cfg4a the 100k
+    CP2K-sized DGEMM batch, cfg5 the convolution layers (per layer and mode:
+    stored-im2col forward / fused backward and implicit forward / backward
+    times, roof fractions, max relative errors)."""
+    out = {}
+    t0 = time.perf_counter()
+    h = bench_hello(args, dev)
+    out["cfg1_hello_80_nt"] = {k: h[k] for k in ("value", "unit", "sync_api_us", "host_pointer_api_us", "gflops",
+                                                  "violations", "max_rel")}
+    log(f"cfg1: {out['cfg1_hello_80_nt']}")
+    sw = bench_sweep(args, dev)
+    rows = sw["sizes"]
+    out["cfg2_sweep"] = {"n": [r["n"] for r in rows],
+                         **{f"{t}_us": [r[t]["us"] for r in rows] for t in ("NN", "NT", "TN", "TT")},
+                         "violations": sum(r[t]["viol"] for r in rows for t in ("NN", "NT", "TN", "TT")),
+                         "cpu_us_NN": [c["us"] for c in sw["cpu_baseline"]["sizes"]] if sw["cpu_baseline"] else None,
+                         "cpu_cores": sw["cpu_baseline"]["cores"] if sw["cpu_baseline"] else None}
+    log(f"cfg2: n=128 NN {rows[-1]['NN']['us']} us, violations {out['cfg2_sweep']['violations']}")
+    mp = bench_mp(args, dev)
+    out["cfg3_mp"] = {"gflops_bf16x3": mp["value"], "gflops_fp32": mp["fp32_gflops"],
+                      "columns": "M,N,K,tb,fp32_us,fp32_roof_frac,fp32_max_rel,bf16x3_us,bf16x3_roof_frac,"
+                                 "bf16x3_max_rel",
+                      "max_rel_fp32": max(r["fp32"]["max_rel"] for r in mp["shapes"]),
+                      "mean_rel_fp32": r3(sum(r["fp32"]["mean_rel"] for r in mp["shapes"]) / len(mp["shapes"])),
+                      "max_rel_bf16x3": max(r["bf16x3"]["max_rel"] for r in mp["shapes"]),
+                      "mean_rel_bf16x3": r3(sum(r["bf16x3"]["mean_rel"] for r in mp["shapes"]) / len(mp["shapes"])),
+                      "rows": [[r["M"], r["N"], r["K"], r["tb"], round(r["fp32"]["ms"] * 1e3, 1),
+                                r["fp32"]["roof_frac"], r3(r["fp32"]["max_rel"]), round(r["bf16x3"]["ms"] * 1e3, 1),
+                                r["bf16x3"]["roof_frac"], r3(r["bf16x3"]["max_rel"])] for r in mp["shapes"]]}
+    log(f"cfg3: bf16x3 {mp['value']} GFLOP/s, fp32 {mp['fp32_gflops']} GFLOP/s")
+    dg = bench_dgemm_batched(args, dev)
+    out["cfg4a_dgemm_batch"] = {"gflops": dg["value"], "ms_per_step": dg["ms_per_step"], "hbm_gbs": dg["hbm_gbs"],
+                                "hbm_frac": dg["roofline"]["frac"], "max_rel_err": dg["max_rel_err"],
+                                "cpu_baseline": dg["cpu_baseline"]}
+    log(f"cfg4a: {out['cfg4a_dgemm_batch']}")
+    cv = bench_conv(args, dev)
+    keys = ("fwd_ms", "fwd_roof_frac", "bwd_ms", "bwd_roof_frac", "implicit_fwd_ms", "implicit_fwd_roof_frac",
+            "implicit_bwd_ms", "fwd_max_rel", "dW_max_rel", "dX_max_rel", "implicit_dX_max_rel",
+            "implicit_dW_max_rel")
+    out["cfg5_conv"] = {"gflops_bf16x3": cv["value"], "gflops_fp32": cv["fp32_gflops"], "columns": ",".join(keys),
+                        "layers": {f"{l['M']}x{l['N']}x{l['K']}": {m: [l[m][k] for k in keys]
+                                                                  for m in ("fp32", "bf16x3")}
+                                   for l in cv["layers"]}}
+    log(f"cfg5: bf16x3 {cv['value']} GFLOP/s, fp32 {cv['fp32_gflops']} GFLOP/s")
+    out["wall_s"] = round(time.perf_counter() - t0, 1)
+    return out
 
 
 OTHER = {"conv": bench_conv, "mp": bench_mp, "sweep": bench_sweep, "hello": bench_hello,

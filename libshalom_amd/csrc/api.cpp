@@ -96,7 +96,8 @@ constexpr int kProbeBlocks = 64;
 __device__ int g_xcc_probe[kProbeBlocks];
 
 // Whether the device deals the workgroups of a 1-D launch to its 8 XCDs
-// round-robin: a launch of kProbeBlocks workgroups records each one's XCD.
+// round-robin (from a start that depends on the queue and its history): a
+// launch of kProbeBlocks workgroups records each one's XCD.
 // Any failure (or a stream capture in progress) answers no.
 bool probe_xcd_round_robin()
 {
@@ -111,8 +112,9 @@ bool probe_xcd_round_robin()
         (void)hipGetLastError();
         return false;
     }
+    // round-robin from some start: (XCD - workgroup) mod 8 the same for all
     for (int i = 0; i < kProbeBlocks; ++i)
-        if (h[i] != i % 8) return false;
+        if (h[i] < 0 || h[i] > 7 || (h[i] - i + 64) % 8 != (h[0] + 64) % 8) return false;
     return true;
 }
 
@@ -203,7 +205,7 @@ struct FixupSpace {
 };
 std::mutex g_fx_mu;
 std::map<std::pair<int, hipStream_t>, FixupSpace> g_fx;
-constexpr size_t kCounterBytes = 1 << 20;  // 256 K tile counters
+constexpr size_t kCounterBytes = 1 << 20;  // counters of 128 K tiles (two ints each)
 // Partial tiles a single GEMM's split-K may use (the planner picks fewer K
 // slices when more would be needed).
 constexpr size_t kSplitKSlotLimit = (size_t)256 << 20;
@@ -315,7 +317,7 @@ void forced_plan(int prec, long M, long N, long K, SinglePlan *p)
     const long kper = ks == 1 ? std::max<long>(K, 1) : cdiv(cdiv(K, ks), kd) * kd;
     const int nks = ks == 1 ? 1 : (int)cdiv(K, kper);
     const TileShape ts = kTileShapes[t];
-    if (nks > 1 && ((size_t)cdiv(M, ts.bm) * cdiv(N, ts.bn) > kCounterBytes / sizeof(int) ||
+    if (nks > 1 && ((size_t)cdiv(M, ts.bm) * cdiv(N, ts.bn) > kCounterBytes / (2 * sizeof(int)) ||
                     splitk_slot_bytes(t, M, N, nks, esize) > kSplitKSlotLimit))
         return;
     *p = SinglePlan{t, nks, (int)kper};
@@ -332,7 +334,7 @@ void forced_plan(int prec, long M, long N, long K, SinglePlan *p)
 // when that address is taken, and every launch leaves them zero again.
 // Calls that overlap on one GPU share the array and race.
 // ---------------------------------------------------------------------------
-constexpr size_t kNoAllocCounterBytes = (size_t)64 << 10;  // 16 K tile counters
+constexpr size_t kNoAllocCounterBytes = (size_t)64 << 10;  // counters of 8 K tiles
 constexpr size_t kNoAllocSlotBytes = (size_t)32 << 20;     // partial tiles
 __device__ __align__(256) char scratch[kNoAllocCounterBytes + kNoAllocSlotBytes];
 std::mutex g_noalloc_mu;
@@ -383,7 +385,7 @@ int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B,
     // split-K needs the XCD placement of its slices (k_gemm_tiles); without
     // it the agent-scope fences make a split slower than none
     const size_t slot_limit = noalloc ? kNoAllocSlotBytes : kSplitKSlotLimit;
-    const size_t counter_limit = (noalloc ? kNoAllocCounterBytes : kCounterBytes) / sizeof(int);
+    const size_t counter_limit = (noalloc ? kNoAllocCounterBytes : kCounterBytes) / (2 * sizeof(int));
     SinglePlan p = plan_single(prec, M, N, K, mp, info.num_cu, slot_limit, info.xcd_rr ? counter_limit : 0);
     if (info.xcd_rr && !noalloc) forced_plan(prec, M, N, K, &p);
     TilesLaunch fn = launch_table().tiles[prec][ta][tb][p.tile];
@@ -555,7 +557,7 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
                 }
             }
             const bool traffic_ok = 2.0 * slot_bytes <= 0.25 * ops;
-            if (traffic_ok && slot_bytes <= ((size_t)256 << 20) && counters * sizeof(int) <= kCounterBytes) break;
+            if (traffic_ok && slot_bytes <= ((size_t)256 << 20) && counters * 2 * sizeof(int) <= kCounterBytes) break;
             len *= 2;
         }
     };
@@ -594,7 +596,7 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
         if (P.nslices <= 1) continue;
         P.counters = cnt;
         P.slots = slots;
-        cnt += (size_t)(P.sO == 0 ? 1 : P.nimg) * P.ntiles;
+        cnt += 2 * (size_t)(P.sO == 0 ? 1 : P.nimg) * P.ntiles;  // two ints per tile (splitk_arrive)
         slots += (size_t)P.units * T * T;
     }
     return T;

@@ -41,7 +41,7 @@ struct BwdProblem {
                           // tiles padded to a multiple of 8 (unit_of)
     int xcd;              // slices of a tile share an XCD (see splitk_arrive)
     float *slots;         // nslices > 1: partial tiles, [tile][slice][BM * BN]
-    int *counters;        // nslices > 1: arrivals per tile (zero between launches)
+    int *counters;        // nslices > 1: two ints per tile (splitk_arrive; zero between launches)
     // A convolution output with the batch folded into its columns (fold_total
     // > 0): column n of the R x Cc output is pixel n % ldo of image n / ldo,
     // at n + (n / ldo) * (R - 1) * ldo; fold_total = all output elements.
@@ -108,7 +108,8 @@ struct SliceSrc {
 template <int BM, int BN, int NT>
 __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_out, int r0, int c0, int *flag)
 {
-    if (!splitk_arrive(P.counters, tg, P.nslices, P.xcd != 0, flag)) return;
+    const int f = splitk_arrive(P.counters, tg, P.nslices, P.xcd != 0, flag);
+    if (f == 0) return;
     const int tr = min(BM, P.R - r0), tcn = min(BN, P.Cc - c0);
     const float *base = P.slots + (size_t)tg * P.nslices * (BM * BN);
     float *O = (float *)P.O + (long long)img_out * P.sO;
@@ -118,7 +119,8 @@ __device__ __forceinline__ void bwd_fixup(const BwdProblem &P, int tg, int img_o
         // slices summed in slice order; eight loads in flight at a time (a
         // load-then-add loop put one memory round trip per slice in series)
         f32x4 acc = *(const f32x4 *)(base + e);
-        int s = 1;
+        if (f == 3) acc = f32x4{NAN, NAN, NAN, NAN};  // partial tiles not to be trusted
+        int s = f == 3 ? P.nslices : 1;
         for (; s + 8 <= P.nslices; s += 8) {
             f32x4 t[8];
 #pragma unroll

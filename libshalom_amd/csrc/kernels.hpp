@@ -57,11 +57,13 @@ __device__ __forceinline__ void splitk_sum(const T *__restrict__ slots, int ks, 
 // register XCC_ID (id 20), bits 3:0.
 __device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & 15; }
 
-// Workgroup linear ids of a split-K launch: the dispatcher deals workgroups
-// to the 8 XCDs round-robin (workgroup L to XCD L % 8, verified on the device
-// before such a launch is used), so workgroup L = ((t / 8) * ks + s) * 8 +
-// t % 8 runs slice s of tile t on XCD t % 8: every slice of a tile shares
-// one L2.  Returns the tile, or -1 for the padding past the last tile.
+// Workgroup linear ids of a split-K launch.  The dispatcher deals the
+// workgroups of a 1-D launch to the 8 XCDs round-robin: workgroup L runs on
+// XCD (L + o) % 8, where the start o depends on the hardware queue and its
+// history (verified on the device before such a launch is used).  So
+// workgroup L = ((t / 8) * ks + s) * 8 + t % 8 runs slice s of tile t on XCD
+// (t + o) % 8 for every s: the slices of a tile share one L2.  Returns the
+// tile, or -1 for the padding past the last tile.
 __device__ __forceinline__ int splitk_tile(int L, int ks, int ntiles, int *slice)
 {
     const int t = (L / (8 * ks)) * 8 + (L & 7);
@@ -70,17 +72,19 @@ __device__ __forceinline__ int splitk_tile(int L, int ks, int ntiles, int *slice
 }
 
 // After a slice has written its partial tile: publish it and count the
-// arrival at tile t; returns (to every thread) whether this workgroup is the
-// last slice, which then sees every slice's partial tile.  A slice on the
-// tile's XCD waits for its stores (the L1 is write-through, so they are in
-// the XCD's L2, where the other slices read them); one found elsewhere
-// writes its L2 back (agent-scope release) and marks the tile, whose last
-// slice then invalidates its caches (agent-scope acquire) before reading.
-// The counter returns to zero for the next launch.
-__device__ __forceinline__ bool splitk_arrive(int *counters, int t, int ks, bool xcd_local, int *flag)
+// arrival at tile t (counters[2t]: arrivals; counters[2t + 1]: the XCDs the
+// slices ran on, one bit each); returns to every thread 0 unless this
+// workgroup is the last slice, which then sees every slice's partial tile.
+// With xcd_local (slices of a tile share an XCD, splitk_tile), a slice waits
+// for its stores -- the L1 is write-through, so they are in the XCD's L2,
+// where the last slice reads them -- and the last slice returns 2; should
+// the slices have run on different XCDs after all, it returns 3 and the
+// caller must not trust the partial tiles.  Otherwise every slice writes its
+// L2 back and the last one invalidates its caches (agent-scope release and
+// acquire) and returns 1.  The words return to zero for the next launch.
+__device__ __forceinline__ int splitk_arrive(int *counters, int t, int ks, bool xcd_local, int *flag)
 {
-    const bool local = xcd_local && xcc_id() == (t & 7);
-    if (local) {
+    if (xcd_local) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_s_waitcnt(0);
     } else {
@@ -88,11 +92,17 @@ __device__ __forceinline__ bool splitk_arrive(int *counters, int t, int ks, bool
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const int old = atomicAdd(&counters[t], local ? 1 : 0x10001);
-        const bool last = (old & 0xffff) == ks - 1;
-        // 1: last, agent-scope acquire needed; 2: last, on the tile's XCD with every slice
-        *flag = !last ? 0 : (!local || (old >> 16) != 0) ? 1 : 2;
-        if (last) counters[t] = 0;
+        const int bit = 1 << xcc_id();
+        atomicOr(&counters[2 * t + 1], bit);
+        const bool last = atomicAdd(&counters[2 * t], 1) == ks - 1;
+        int f = 0;
+        if (last) {
+            const int xcds = atomicOr(&counters[2 * t + 1], 0);
+            f = !xcd_local ? 1 : xcds == bit ? 2 : 3;
+            counters[2 * t] = 0;
+            counters[2 * t + 1] = 0;
+        }
+        *flag = f;
     }
     __syncthreads();
     const int f = *flag;
@@ -100,7 +110,15 @@ __device__ __forceinline__ bool splitk_arrive(int *counters, int t, int ks, bool
         __threadfence();
     else if (f == 2)
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    return f != 0;
+    return f;
+}
+
+// The tr x tc tile of C at C (leading dimension ldc) set to NaN: the loud
+// result of a split reduction whose partial tiles could not be trusted.
+template <typename T, int NT>
+__device__ __forceinline__ void nan_tile(T *C, int ldc, int tr, int tc)
+{
+    for (int e = threadIdx.x; e < tr * tc; e += NT) C[(size_t)(e / tc) * ldc + e % tc] = __builtin_nan("");
 }
 
 // One GEMM over a grid of tiles x K slices.  Without split-K (ks == 1)
@@ -110,7 +128,8 @@ __device__ __forceinline__ bool splitk_arrive(int *counters, int t, int ks, bool
 // 438-459; here the slices run in parallel), the slices of a tile run on one
 // XCD (splitk_tile), every slice writes its partial tile to its slot, and
 // the last slice to arrive (splitk_arrive) sums the slots in slice order --
-// deterministic, in the same launch -- and applies alpha / beta.
+// deterministic, in the same launch -- and applies alpha / beta.  Counters
+// take two ints per tile (splitk_arrive).
 //
 // Tiles in row-major order: the dispatcher deals workgroups to the 8 XCDs
 // round-robin, so the resident tiles of one XCD are every 8th of a band of
@@ -144,10 +163,15 @@ __global__ __launch_bounds__(Cfg::NT) void k_gemm_tiles(GemmDesc d, int tiles_n,
     t.alpha = alpha;
     t.beta = beta;
     t.stream(src, 0, 1, lds);
-    if (ks == 1 || !splitk_arrive(counters, b, ks, xcd_local != 0, &flag)) return;
-    splitk_sum<T, Cfg::BM, Cfg::BN, Cfg::NT>(slots + (size_t)b * ks * (Cfg::BM * Cfg::BN), ks,
-                                             (T *)d.C + (size_t)m0 * d.ldc + n0, d.ldc, min(Cfg::BM, d.M - m0),
-                                             min(Cfg::BN, d.N - n0), alpha, beta);
+    if (ks == 1) return;
+    const int f = splitk_arrive(counters, b, ks, xcd_local != 0, &flag);
+    if (f == 0) return;
+    T *Ct = (T *)d.C + (size_t)m0 * d.ldc + n0;
+    if (f == 3)
+        nan_tile<T, Cfg::NT>(Ct, d.ldc, min(Cfg::BM, d.M - m0), min(Cfg::BN, d.N - n0));
+    else
+        splitk_sum<T, Cfg::BM, Cfg::BN, Cfg::NT>(slots + (size_t)b * ks * (Cfg::BM * Cfg::BN), ks, Ct, d.ldc,
+                                                 min(Cfg::BM, d.M - m0), min(Cfg::BN, d.N - n0), alpha, beta);
 }
 
 // Records the XCD of each of n workgroups (see xcc_id).

@@ -142,7 +142,7 @@ inline double plan_cost(int prec, long M, long N, long K, int tile, int ksplit, 
 // estimated cost, with at most max_slot_bytes of partial tiles and
 // max_split_tiles tiles when K is split (0: no split).
 inline SinglePlan plan_single(int prec, long M, long N, long K, bool /*mp*/, int num_cu,
-                              size_t max_slot_bytes = (size_t)256 << 20, size_t max_split_tiles = 1 << 18)
+                              size_t max_slot_bytes = (size_t)256 << 20, size_t max_split_tiles = 1 << 17)
 {
     SinglePlan best{T32x32, 1, (int)std::max<long>(K, 1)};
     double best_cost = 1e300;

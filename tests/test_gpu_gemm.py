@@ -249,6 +249,25 @@ def test_fp32_bound_at_tiny_k(dev, prec, K):
     assert bad == 0, st
 
 
+def test_split_k_after_graph_launch(dev):
+    """Split-K GEMMs keep a tile's K slices on one XCD; the dispatcher's
+    round-robin start shifts after a graph with forked branches has run (a
+    batched plan), so run one first, then split-K GEMMs in both float modes,
+    C prefilled with NaN, every element against the oracle."""
+    shapes = [(m, n, k) for m in (9, 70, 130) for n in (5, 40, 100) for k in (3, 50, 120)]
+    _run_batched(dev, shapes, 0, 0, ls.FP32, seed=11)
+    for prec in (ls.FP32, ls.BF16X3):
+        for (M, N, K, tb) in [(4096, 16, 1024, 0), (16, 4096, 1024, 1), (2048, 48, 768, 0)]:
+            assert ls.query_plan(prec, M, N, K)["ksplit"] > 1
+            r = oracle.Drand48(M + K)
+            A = r.matrix(M, K)
+            B = r.matrix(*oracle.b_shape(tb, N, K))
+            dC = nan_c(M, N, dev)
+            ls.sgemm_mp_ex(0, tb, dC, dev_tensor(A, dev), dev_tensor(B, dev), M, N, K, prec)
+            bad, st = check_f32(dC.cpu().numpy(), A, B, 0, tb, M, N, K, prec)
+            assert bad == 0, (prec, M, N, K, st)
+
+
 @pytest.mark.parametrize("ta,tb", TRANS)
 def test_dgemm(dev, ta, tb):
     """CP2K block shapes (benchmark/CP2K_kernels:74-76) plus the M % 7 in
