@@ -847,31 +847,15 @@ struct Tile {
         const int r = lane & 15, q = lane >> 4;
         if (!live) return;
         if constexpr (!Cfg::SPLIT) {
-            constexpr int NS = Cfg::NSLOT;
-            const T *As = (const T *)buf;
-            const T *Bs = (const T *)(buf + Cfg::A_BYTES);
-            T af[WM][NS];
-#pragma unroll
-            for (int a = 0; a < WM; ++a) read_frag<typename Cfg::PA>(As, wr0 + a, r, q, af[a]);
-#pragma unroll
-            for (int b = 0; b < WN; ++b) {
-                if (b >= wn) break;
-                T bf[NS];
-                read_frag<typename Cfg::PB>(Bs, wc0 + b, r, q, bf);
-#pragma unroll
-                for (int s = 0; s < NS; ++s) {
-                    if (s < ksteps) {
-#pragma unroll
-                        for (int a = 0; a < WM; ++a) {
-                            if (a >= wm) break;
-                            if constexpr (sizeof(T) == 4)
-                                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(af[a][s], bf[s], acc[a][b], 0, 0, 0);
-                            else
-                                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[a][s], bf[s], acc[a][b], 0, 0, 0);
-                        }
-                    }
-                }
-            }
+            // FP32 / FP64: the MFMAs of the sub-tiles that reach into the
+            // job, with uniform branches per 16-column block and per row
+            // block rather than per MFMA.  Every k step of the chunk is
+            // multiplied: past K the panels hold zeros, and testing each
+            // step (a branch per MFMA) measured slower than the extra MFMAs
+            // of a job's last chunk; a second, tested copy of the loop for
+            // the last chunk did not fit the persistent kernel's registers.
+            (void)ksteps;
+            compute_float(buf, r, q);
         } else {
             constexpr int NP = Cfg::NP, LDKH = Cfg::PA::LDKH;
             static_assert(Cfg::PB::LDKH == LDKH, "one split row length");
@@ -906,6 +890,45 @@ struct Tile {
                 }
             }
         }
+    }
+
+    __device__ __forceinline__ void compute_float(const char *__restrict__ buf, int r, int q)
+    {
+        constexpr int NS = Cfg::NSLOT;
+        const T *As = (const T *)buf;
+        const T *Bs = (const T *)(buf + Cfg::A_BYTES);
+        T af[WM][NS];
+#pragma unroll
+        for (int a = 0; a < WM; ++a) read_frag<typename Cfg::PA>(As, wr0 + a, r, q, af[a]);
+        const bool all_rows = wm == WM;
+#pragma unroll
+        for (int b = 0; b < WN; ++b) {
+            if (b >= wn) break;
+            T bf[NS];
+            read_frag<typename Cfg::PB>(Bs, wc0 + b, r, q, bf);
+            if (all_rows) {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int a = 0; a < WM; ++a) mfma_step(acc[a][b], af[a][s], bf[s]);
+            } else {
+#pragma unroll
+                for (int s = 0; s < NS; ++s)
+#pragma unroll
+                    for (int a = 0; a < WM; ++a) {
+                        if (a >= wm) break;
+                        mfma_step(acc[a][b], af[a][s], bf[s]);
+                    }
+            }
+        }
+    }
+
+    __device__ __forceinline__ static void mfma_step(Acc &c, T a, T b)
+    {
+        if constexpr (sizeof(T) == 4)
+            c = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+        else
+            c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
 
     template <class StageT>
