@@ -49,7 +49,11 @@ typedef void *LibShalom_stream_t;
  * lo*hi, FP32 accumulation.  BF16: one BF16 product.  BF16X6: three-way
  * split (exact for every FP32 value), the eight products of the pieces but
  * lo2*lo2, FP32 accumulation: within |C - C64| <= 2*K*2^-24*(|A||B|), as
- * FP32.  FP64 GEMMs always use FP64 MFMA. */
+ * FP32.  In the split modes every result element that comes out Inf or NaN
+ * (an Inf or NaN operand, or a finite one beyond the largest bf16) is
+ * recomputed in FP32 by the same launch, so special values give the
+ * classes (finite, +-Inf, NaN) of the FP32 mode.  FP64 GEMMs always use
+ * FP64 MFMA. */
 #define LIBSHALOM_FP32 0
 #define LIBSHALOM_BF16X3 1
 #define LIBSHALOM_BF16 2

@@ -61,6 +61,7 @@ struct BwdProblem {
 template <class Cfg, int IM2COL_ = kPlainB>
 struct SliceSrc {
     static constexpr int IM2COL = IM2COL_;
+    static constexpr int REPAIR = kRepairByCaller;
     const void *L, *Rt;
     long long sL, sR;
     int lda, ldb;
@@ -156,17 +157,27 @@ __device__ __forceinline__ Unit unit_of(int nslices, int units, int u)
     return Unit{tg < units / nslices ? tg : -1, (u >> 3) % nslices};
 }
 
-template <class Cfg, int IM2COL = kPlainB>
-__device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, int *flag, const ConvGeom &geom = {})
+// The job source of workgroup u of problem P (the tile group and slice of
+// unit_of), or tg < 0 for padding past the last tile group.
+template <class Cfg, int IM2COL>
+struct UnitSrc {
+    SliceSrc<Cfg, IM2COL> src;
+    int tg, img_out;
+};
+template <class Cfg, int IM2COL>
+__device__ __forceinline__ UnitSrc<Cfg, IM2COL> unit_src(const BwdProblem &P, int u, const ConvGeom &geom)
 {
+    UnitSrc<Cfg, IM2COL> us;
     const Unit un = unit_of(P.nslices, P.units, u);
-    if (un.tg < 0) return;
+    us.tg = un.tg;
+    if (un.tg < 0) return us;
     const int tg = un.tg, s = un.s;
     const bool shared = P.sO == 0;
     const int img_out = shared ? 0 : tg / P.ntiles;
+    us.img_out = img_out;
     const int t = shared ? tg : tg - img_out * P.ntiles;
     const int tr = t / P.tiles_c, tc = t - tr * P.tiles_c;
-    SliceSrc<Cfg, IM2COL> src;
+    SliceSrc<Cfg, IM2COL> &src = us.src;
     src.geom = geom;
     src.L = P.L;
     src.Rt = P.Rt;
@@ -218,9 +229,60 @@ __device__ __forceinline__ void bwd_unit(const BwdProblem &P, int u, char *lds, 
         src.out = P.slots + ((size_t)tg * P.nslices + s) * (Cfg::BM * Cfg::BN);
         src.ldo = Cfg::BN;
     }
+    return us;
+}
+
+// The split modes' FP32 recompute of the Inf / NaN elements of unit u's
+// tile once its stream has stored it (Tile::rescan; a rare path).  The
+// problem and the geometry are read again from the kernel's argument
+// segment (p_off, g_off: their byte offsets there) through a pointer the
+// compiler cannot see through: reusing the registers that hold them would
+// keep those live through the stream loop and raise the kernels' register
+// counts.
+typedef const __attribute__((address_space(4))) char *KernArgs;
+
+template <class Cfg, int IM2COL>
+__device__ __forceinline__ void bwd_rescan(int p_off, int g_off, int u)
+{
+#if __HIP_DEVICE_COMPILE__  // (the host pass only parses device code)
+    typedef const __attribute__((address_space(4))) BwdProblem *PP;
+    typedef const __attribute__((address_space(4))) ConvGeom *GP;
+    KernArgs args = (KernArgs)__builtin_amdgcn_kernarg_segment_ptr();
+    asm volatile("" : "+s"(args));
+    const BwdProblem P = *(PP)(args + p_off);
+    ConvGeom g{};
+    if constexpr (IM2COL != kPlainB) g = *(GP)(args + g_off);
+    const UnitSrc<Cfg, IM2COL> us = unit_src<Cfg, IM2COL>(P, u, g);
+    Tile<Cfg>::rescan(us.src, us.src.nseg - 1, us.src.nseg - 1, 1);
+#endif
+}
+
+// Byte offsets of the arguments of the kernels below in their argument
+// segment (laid out in order, each at its natural alignment).
+struct ArgsXY {
+    BwdProblem X, Y;
+};
+struct ArgsPG {
+    BwdProblem P;
+    ConvGeom g;
+};
+struct ArgsXYGG {
+    BwdProblem X, Y;
+    ConvGeom gx, gy;
+};
+
+// Unit u of problem P, the kernel argument at byte offset p_off (geom: the
+// one at g_off).
+template <class Cfg, int IM2COL = kPlainB>
+__device__ __forceinline__ void bwd_unit(const BwdProblem &P, int p_off, int u, char *lds, int *flag,
+                                         const ConvGeom &geom = {}, int g_off = 0)
+{
+    const UnitSrc<Cfg, IM2COL> us = unit_src<Cfg, IM2COL>(P, u, geom);
+    if (us.tg < 0) return;
+    const SliceSrc<Cfg, IM2COL> &src = us.src;
     Tile<Cfg> tile;
-    tile.stream(src, 0, 1, lds);
-    if (P.nslices > 1) bwd_fixup<Cfg::BM, Cfg::BN, Cfg::NT>(P, tg, img_out, src.r0, src.c0, flag);
+    if (tile.stream(src, 0, 1, lds)) bwd_rescan<Cfg, IM2COL>(p_off, g_off, u);
+    if (P.nslices > 1) bwd_fixup<Cfg::BM, Cfg::BN, Cfg::NT>(P, us.tg, us.img_out, src.r0, src.c0, flag);
 }
 
 template <class CX, class CY>
@@ -238,9 +300,9 @@ k_gemm_backward(BwdProblem X, BwdProblem Y)
     __shared__ int flag;
     const int u = blockIdx.x, xg = (X.grid + 7) & ~7;  // Y starts on a multiple of 8
     if (u < xg)
-        bwd_unit<CX>(X, u, lds, &flag);
+        bwd_unit<CX>(X, 0, u, lds, &flag);
     else
-        bwd_unit<CY>(Y, u - xg, lds, &flag);
+        bwd_unit<CY>(Y, offsetof(ArgsXY, Y), u - xg, lds, &flag);
 }
 
 // One problem on its own: a strided batch of GEMMs C_i = op(A_i) op(B_i)
@@ -252,7 +314,7 @@ k_gemm_strided(BwdProblem P)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
     __shared__ int flag;
-    bwd_unit<Cfg>(P, blockIdx.x, lds, &flag);
+    bwd_unit<Cfg>(P, 0, blockIdx.x, lds, &flag);
 }
 
 // Convolution forward as an implicit GEMM: image i's output (Cout x P*Q) is
@@ -265,7 +327,7 @@ k_conv_fwd(BwdProblem P, ConvGeom g)
 {
     __shared__ __attribute__((aligned(16))) char lds[Cfg::LDS_BYTES];
     __shared__ int flag;
-    bwd_unit<Cfg, kIm2col>(P, blockIdx.x, lds, &flag, g);
+    bwd_unit<Cfg, kIm2col>(P, 0, blockIdx.x, lds, &flag, g, offsetof(ArgsPG, g));
 }
 
 // Convolution backward as two implicit GEMMs in ONE launch.  X units: the
@@ -283,9 +345,9 @@ k_conv_backward(BwdProblem X, BwdProblem Y, ConvGeom gx, ConvGeom gy)
     __shared__ int flag;
     const int u = blockIdx.x, xg = (X.grid + 7) & ~7;  // Y starts on a multiple of 8
     if (u < xg)
-        bwd_unit<CX, kIm2col>(X, u, lds, &flag, gx);
+        bwd_unit<CX, kIm2col>(X, 0, u, lds, &flag, gx, offsetof(ArgsXYGG, gx));
     else
-        bwd_unit<CY, IMY>(Y, u - xg, lds, &flag, gy);
+        bwd_unit<CY, IMY>(Y, offsetof(ArgsXY, Y), u - xg, lds, &flag, gy, offsetof(ArgsXYGG, gy));
 }
 
 // ---------------------------------------------------------------------------

@@ -128,6 +128,15 @@ struct FoldedCols {
     __device__ __forceinline__ int extent(int, int, int) const { return ext; }
 };
 
+// Where a split-mode stream recomputes its Inf / NaN results in FP32 (see
+// Tile::any_nonfinite): at the store of the tile, leaving those elements out
+// of it (the one C tile of a single-job stream, which may scale by alpha and
+// beta); after the stream, reading back the stored tiles of the jobs that
+// held one (an out-of-line call); or by the stream's caller, told so by
+// stream()'s result.  Each source takes the one that left its kernels'
+// register counts as they were.
+enum RepairAt : int { kRepairAtStore = 0, kRepairAfterCall = 1, kRepairByCaller = 2 };
+
 // GemmDesc::flags bit: the job is one segment of a longer reduction.  Its C
 // tile is not stored and the next job of the same stream (which must have the
 // same tile) adds to the same accumulators.
@@ -1049,7 +1058,7 @@ struct Tile {
     __device__ __forceinline__ bool step(Cursor &cur, Cursor &n1, Cursor &n2, StageA &a0, SB &b0, StageA &a1,
                                          SB &b1, const PX &px, char *lds, int bi, const Src &src,
                                          int stride, int tid,
-                                         int lane, int wave)
+                                         int lane, int wave, int &bad_lo, int &bad_hi)
     {
         load_chunk(a0, b0, src, px, src.get(max(n2.job, 0)), n2.c * BK, n2.job >= 0, tid);
         // keep the loads ahead of the multiply in the schedule
@@ -1066,7 +1075,30 @@ struct Tile {
         if constexpr (Cfg::NBUF == 1) __syncthreads();
         if constexpr (Src::IM2COL == kIm2colTR) b1.fix_runs();
         chunk_to_lds(a1, b1, lds_buf(lds, bi + 1), tid);
-        if (cur.c + 1 == cur.nch && !chained) store((T *)dc.C, dc.ldc, dc.M, dc.N, src.cols());
+        if (cur.c + 1 == cur.nch && !chained) {
+            if constexpr (Cfg::SPLIT && Src::REPAIR == kRepairAtStore) {
+                // the stream's one C tile: its Inf / NaN elements are left
+                // out of the store (keeping C for beta) and recomputed right
+                // away
+                if (__builtin_expect(any_nonfinite(), 0)) {
+                    const unsigned long long bad = nonfinite_mask();
+                    store((T *)dc.C, dc.ldc, dc.M, dc.N, src.cols(), bad);
+                    repair(src, cur.job, stride, bad, src.cols());
+                } else {
+                    store((T *)dc.C, dc.ldc, dc.M, dc.N, src.cols());
+                }
+            } else {
+                // a job stream (no alpha / beta): the job's number is noted
+                // and its Inf / NaN elements recomputed after the stream
+                if constexpr (Cfg::SPLIT) {
+                    if (__builtin_expect(any_nonfinite(), 0)) {
+                        bad_lo = min(bad_lo, cur.job);
+                        bad_hi = cur.job;
+                    }
+                }
+                store((T *)dc.C, dc.ldc, dc.M, dc.N, src.cols());
+            }
+        }
         if (n1.job < 0) return false;
         // n1 starting a job means cur ends one: a chained job keeps its sums
         if (n1.c == 0 && !chained) {
@@ -1086,13 +1118,16 @@ struct Tile {
     // stream: the chunks of the next job are loaded while the current job's
     // last chunks are multiplied and its C tile written, so loads stay in
     // flight instead of draining at every job.
+    //
+    // Returns whether the stream stored an Inf or NaN that its caller is to
+    // recompute (split modes, sources with REPAIR == kRepairByCaller).
     template <class Src>
-    __device__ __forceinline__ void stream(const Src &src, int first, int stride, char *lds)
+    __device__ __forceinline__ bool stream(const Src &src, int first, int stride, char *lds)
     {
         const int tid = threadIdx.x;
         const int lane = tid & (kWave - 1);
         const int wave = uniform(tid >> 6);
-        if (first >= src.count()) return;
+        if (first >= src.count()) return false;
         const GemmDesc d0 = src.get(first);
         Cursor cur;
         cur.job = first;
@@ -1111,9 +1146,198 @@ struct Tile {
         if constexpr (Src::IM2COL == kIm2colTR) b0.fix_runs();
         chunk_to_lds(a0, b0, lds_buf(lds, 0), tid);
         __syncthreads();
+        int bad_lo = 0x7fffffff, bad_hi = -1;  // jobs of the stream holding an Inf / NaN (split modes)
         for (;;) {
-            if (!step(cur, n1, n2, a0, b0, a1, b1, px, lds, 0, src, stride, tid, lane, wave)) break;
-            if (!step(cur, n1, n2, a1, b1, a0, b0, px, lds, 1, src, stride, tid, lane, wave)) break;
+            if (!step(cur, n1, n2, a0, b0, a1, b1, px, lds, 0, src, stride, tid, lane, wave, bad_lo, bad_hi)) break;
+            if (!step(cur, n1, n2, a1, b1, a0, b0, px, lds, 1, src, stride, tid, lane, wave, bad_lo, bad_hi)) break;
+        }
+        if constexpr (Cfg::SPLIT && Src::REPAIR == kRepairAfterCall) {
+            if (__builtin_expect(bad_hi >= 0, 0)) rescan_call(src, bad_lo, bad_hi, stride);
+        }
+        return Cfg::SPLIT && Src::REPAIR == kRepairByCaller && bad_hi >= 0;
+    }
+
+    // IEEE special values in the split modes.  The bf16 pieces of an Inf
+    // are not Infs (Inf - Inf makes the later pieces NaN), a finite value
+    // above the largest bf16 rounds its first piece to Inf, and a cross
+    // product of an Inf with a zero or opposite-signed later piece of the
+    // other operand is NaN where FP32's Inf * b is Inf.  In the other
+    // direction no special value is lost: an Inf or NaN operand meets every
+    // piece of its partner, one of which is nonzero unless the partner is 0
+    // or below bf16's range (NaN either way), so an element that FP32 makes
+    // Inf or NaN comes out Inf or NaN here too, up to the summation order
+    // near overflow, which FP32 implementations do not fix either.  So once
+    // a job's sums are complete, the elements that are not finite are left
+    // out of the tile's store and recomputed in FP32 afterwards (each by its
+    // lane alone, FMA over k in order); the others are kept.  Detection is
+    // one FMA per accumulator (x * 0 + s stays 0 unless x is Inf or NaN)
+    // and a wave-wide test.
+
+    // Whether any accumulator of the wave is Inf or NaN (uniform).
+    __device__ __forceinline__ bool any_nonfinite() const
+    {
+        if (!live) return false;
+        float s[2] = {0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < WM; ++a)
+#pragma unroll
+            for (int b = 0; b < WN; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) s[i & 1] = __builtin_fmaf(acc[a][b][i], 0.f, s[i & 1]);
+        const float t = s[0] + s[1];
+        return __builtin_amdgcn_ballot_w64(t != t) != 0;
+    }
+
+    // The lane's Inf / NaN accumulators, bit (a * WN + b) * 4 + i (an
+    // all-ones exponent, tested on the bits: a float compare per element
+    // raised the kernels' register counts).
+    __device__ __forceinline__ unsigned long long nonfinite_mask() const
+    {
+        static_assert(WM * WN * 4 <= 64, "one mask bit per accumulator");
+        unsigned lo = 0, hi = 0;
+#pragma unroll
+        for (int a = 0; a < WM; ++a)
+#pragma unroll
+            for (int b = 0; b < WN; ++b)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int e = (a * WN + b) * 4 + i;
+                    // exponent + 1 reaches 0x100 only for an all-ones exponent
+                    const unsigned x = (((__builtin_bit_cast(unsigned, acc[a][b][i]) >> 23) & 0xffu) + 1u) >> 8;
+                    if (e < 32)
+                        lo |= x << (e & 31);
+                    else
+                        hi |= x << (e & 31);
+                }
+        return ((unsigned long long)hi << 32) | lo;
+    }
+
+    // The FP32 sum over k of element (row, col) of job `job` and the jobs
+    // chained into it, one FMA at a time in k order.
+    template <class Src>
+    __device__ __forceinline__ static float dot_fp32(const Src &src, int j0, int job, int stride, int row, int col)
+    {
+        float v = 0.f;
+        for (int j = j0; j <= job; j += stride) {
+            const GemmDesc d = src.get(j);
+            const float *A = (const float *)d.A;
+#pragma clang loop unroll(disable)
+            for (int k = 0; k < d.K; ++k) {
+                const float x = Cfg::TA ? A[(long long)k * d.lda + row] : A[(long long)row * d.lda + k];
+                v = __builtin_fmaf(x, b_elem(src, d, k, col), v);
+            }
+        }
+        return v;
+    }
+
+    // The first job of the chain that ends with `job`.
+    template <class Src>
+    __device__ __forceinline__ static int chain_start(const Src &src, int job, int stride)
+    {
+        int j0 = job;
+        while (j0 - stride >= 0 && (src.get(j0 - stride).flags & kChain) != 0) j0 -= stride;
+        return j0;
+    }
+
+    // Jobs lo, lo + stride, ..., hi of a stream, already stored: this wave's
+    // elements of their C tiles that are Inf or NaN are recomputed in FP32.
+    // The wave reads back what it stored itself.
+    template <class Src>
+    __device__ __forceinline__ static void rescan(const Src &src, int lo, int hi, int stride)
+    {
+        const auto cm = src.cols();
+        const int lane = threadIdx.x & (kWave - 1);
+        const int r = lane & 15, q = lane >> 4;
+        // the wave's block (as layout() places it), derived afresh so that
+        // nothing stays live across the stream for this rare path
+        const int wave = uniform(threadIdx.x >> 6);
+        const int wr0 = (wave / Cfg::GN) * WM, wc0 = (wave % Cfg::GN) * WN;
+        for (int job = lo; job <= hi; job += stride) {
+            const GemmDesc dj = src.get(job);
+            if (dj.flags & kChain) continue;  // stored with the chain's last job
+            const int j0 = chain_start(src, job, stride);
+#pragma clang loop unroll(disable)
+            for (int e = 0; e < WM * WN * 4; ++e) {
+                const int ab = e >> 2, a = ab / WN, b = ab - a * WN;
+                const int row = (wr0 + a) * 16 + acc_row(q, e & 3), col = (wc0 + b) * 16 + r;
+                if (row >= dj.M || col >= dj.N) continue;
+                float *o = (float *)dj.C + (long long)row * dj.ldc + cm(col);
+                if (!__builtin_isfinite(*o)) *o = dot_fp32(src, j0, job, stride, row, col);
+            }
+        }
+    }
+
+    // rescan() out of line, for the sources whose stream loop it made
+    // take more registers when inlined (nothing of the loop is live where
+    // it runs, but the allocation of the whole kernel changed).
+    template <class Src>
+    __device__ __attribute__((noinline)) static void rescan_call(const Src src, int lo, int hi, int stride)
+    {
+        rescan(src, lo, hi, stride);
+    }
+
+    // The elements of mask `bad` (see nonfinite_mask()) of the job just stored
+    // without them, recomputed in FP32 and written as store() writes them.
+    // The FMA chain runs over the job and the jobs chained into it.
+    template <class Src, class Cols>
+    __device__ __forceinline__ void repair(const Src &src, int job, int stride, unsigned long long bad,
+                                           const Cols &cm) const
+    {
+        const int j0 = chain_start(src, job, stride);
+        const GemmDesc dj = src.get(job);
+        const int lane = threadIdx.x & (kWave - 1);
+        const int r = lane & 15, q = lane >> 4;
+        while (bad != 0) {
+            const int e = __builtin_ctzll(bad);
+            bad &= bad - 1;
+            const int ab = e >> 2, a = ab / WN, b = ab - a * WN;
+            const int row = (wr0 + a) * 16 + acc_row(q, e & 3), col = (wc0 + b) * 16 + r;
+            if (row >= dj.M || col >= dj.N) continue;
+            float v = dot_fp32(src, j0, job, stride, row, col);
+            float *o = (float *)dj.C + (long long)row * dj.ldc + cm(col);
+            if (scale && !Cols::MASK_ROWS) v = beta != T(0) ? alpha * v + beta * *o : alpha * v;
+            *o = v;
+        }
+    }
+
+    // Element (k, col) of job d's right operand, k and col relative to the
+    // job: stored (plain or transposed), or gathered from a convolution's
+    // input as the job's loads do (see Stage::load_im2col / load_im2col_t).
+    template <class Src>
+    __device__ __forceinline__ static float b_elem(const Src &src, const GemmDesc &d, int k, int col)
+    {
+        const float *B = (const float *)d.B;
+        if constexpr (Src::IM2COL == kPlainB) {
+            return Cfg::TB ? B[(long long)col * d.ldb + k] : B[(long long)k * d.ldb + col];
+        } else {
+            const ConvGeom &g = src.geom;
+            int t, n, off = 0;
+            if constexpr (Src::IM2COL == kIm2col) {
+                t = k;             // tap from the job's first channel
+                n = d.ldb + col;   // output pixel (the job's ldb is the tile's first)
+                if (g.fold) {
+                    const int img = udiv(n, g.mPQ);
+                    n -= img * g.PQ;
+                    off = img * g.img;
+                }
+            } else {
+                t = src.c0 + col;  // the stored row is the tap
+                n = src.rho0 + k;  // k runs over output pixels
+            }
+            // the divisions by the geometry's multipliers (a division by a
+            // variable would put its reciprocal setup in registers)
+            const int c = udiv(t, g.mRS), rem = t - c * g.RS, fr = udiv(rem, g.mS), fs = rem - fr * g.S;
+            const int p = udiv(n, g.mQ), pq = n - p * g.Q;
+            int ih = p * g.sh - g.ph + fr * g.dh, iw = pq * g.sw - g.pw + fs * g.dw;
+            if (g.th != 1 || g.tw != 1) {
+                // floor division (ih, iw >= -2^13 > -1024 * stride)
+                const int a = udiv(ih + 1024 * g.th, g.mTh) - 1024, b = udiv(iw + 1024 * g.tw, g.mTw) - 1024;
+                if (ih != a * g.th || iw != b * g.tw) return 0.f;
+                ih = a;
+                iw = b;
+            }
+            if ((unsigned)ih >= (unsigned)g.H || (unsigned)iw >= (unsigned)g.W) return 0.f;
+            return B[off + ((long long)c * g.H + ih) * g.W + iw];
         }
     }
 
@@ -1132,10 +1356,11 @@ struct Tile {
     // Store the accumulators to a row-major OutT matrix (a C tile or a split-K
     // partial) with leading dimension ldo.  The buffer ends right after
     // element (M-1, N-1), so rows >= M fall past it; columns >= N (in range,
-    // they belong to the next tile) take an out-of-range offset.
+    // they belong to the next tile) take an out-of-range offset, and so do
+    // the elements of mask `skip` (bit (a * WN + b) * 4 + i).
     template <typename OutT, class Cols = PlainCols>
     __device__ __forceinline__ void store(OutT *__restrict__ out, int ldo, int M, int N,
-                                          const Cols &cm = Cols{}) const
+                                          const Cols &cm = Cols{}, unsigned long long skip = 0) const
     {
         constexpr int ES_OUT = sizeof(OutT);
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(out, cm.extent(M, ldo, N) * ES_OUT);
@@ -1150,12 +1375,14 @@ struct Tile {
                 const int col = (wc0 + b) * 16 + r;
                 const int row = (wr0 + a) * 16 + acc_row(q, 0);
                 const int off = (row * ldo + cm(col)) * ES_OUT | (col < N ? 0 : kOOB);
+                // element i's offset, out of range when `skip` leaves it out
+                auto at = [&](int i) { return (skip >> ((a * WN + b) * 4 + i)) & 1 ? kOOB : off + i * rstep; };
                 if constexpr (Cols::MASK_ROWS) {
                     // rows past M masked one by one (no alpha / beta here)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         const int ri = (wr0 + a) * 16 + acc_row(q, i);
-                        buf_store1<OutT>(rs, ri < M ? off + i * rstep : kOOB, (OutT)acc[a][b][i]);
+                        buf_store1<OutT>(rs, ri < M ? at(i) : kOOB, (OutT)acc[a][b][i]);
                     }
                 } else if (scale) {
                     OutT old[4] = {OutT(0), OutT(0), OutT(0), OutT(0)};
@@ -1165,12 +1392,12 @@ struct Tile {
                     }
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        buf_store1<OutT>(rs, off + i * rstep,
+                        buf_store1<OutT>(rs, at(i),
                                          beta != T(0) ? (OutT)(alpha * acc[a][b][i] + beta * old[i])
                                                       : (OutT)(alpha * acc[a][b][i]));
                 } else {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) buf_store1<OutT>(rs, off + i * rstep, (OutT)acc[a][b][i]);
+                    for (int i = 0; i < 4; ++i) buf_store1<OutT>(rs, at(i), (OutT)acc[a][b][i]);
                 }
             }
     }
@@ -1179,6 +1406,7 @@ struct Tile {
 // Job source over a device array of jobs.
 struct ArraySrc {
     static constexpr int IM2COL = kPlainB;
+    static constexpr int REPAIR = kRepairAfterCall;
     const GemmDesc *__restrict__ jobs;
     int n;
     __device__ __forceinline__ PlainCols cols() const { return {}; }
@@ -1200,6 +1428,7 @@ struct ArraySrc {
 // Job source of exactly one job held in registers.
 struct OneSrc {
     static constexpr int IM2COL = kPlainB;
+    static constexpr int REPAIR = kRepairAtStore;
     GemmDesc job;
     __device__ __forceinline__ PlainCols cols() const { return {}; }
     __device__ __forceinline__ int count() const { return 1; }
