@@ -564,6 +564,25 @@ def bench_mp(args, dev):
                 f"({row['fp32']['roof_frac']}), bf16x3 {row['bf16x3']['tflops']} TF/s "
                 f"({row['bf16x3']['roof_frac']}), rel err fp32 {row['fp32']['max_rel']} "
                 f"bf16x3 {row['bf16x3']['max_rel']}")
+    # the reference ABI on the 4096 x 16 x 1024 NN shape: device operands
+    # through the synchronous call, and host operands (staged through pinned
+    # bounce buffers, the 16 MiB A dominating)
+    import numpy as np
+    M, N, K = 4096, 16, 1024
+    A = big[:M * K].view(M, K)
+    B = bigB[:K * N].view(K, N)
+    C = Cbuf[:M * N].view(M, N)
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+    Ch = np.empty((M, N), np.float32)
+    ref_api = {}
+    for name, (a, b, c) in (("device_sync_us", (A, B, C)), ("host_pointer_us", (Ah, Bh, Ch))):
+        ls.sgemm_mp(0, 0, c, a, b, M, N, K)
+        t0 = time.perf_counter()
+        for _ in range(20):
+            ls.sgemm_mp(0, 0, c, a, b, M, N, K)
+        ref_api[name] = round((time.perf_counter() - t0) / 20 * 1e6, 1)
+    assert np.array_equal(Ch, C.cpu().numpy())
+    log(f"mp reference API 4096x16x1024: {ref_api}")
     tf = sum(2.0 * r["M"] * r["N"] * r["K"] for r in rows)
     t32 = sum(r["fp32"]["ms"] for r in rows)
     t3 = sum(r["bf16x3"]["ms"] for r in rows)
@@ -571,7 +590,8 @@ def bench_mp(args, dev):
           "value": round(tf / t3 / 1e6, 1), "unit": "GFLOP/s", "fp32_gflops": round(tf / t32 / 1e6, 1),
           "n_gpus": 1, "dtype": "bf16x3 (value) and fp32", "data": "synthetic: U[-1,1)",
           "config": {"workload": "M in {4096,16384,65536} x N in {16,32,64} x K in {64,256,1024} and "
-                                 "the transposed short-wide shapes"}, "shapes": rows})
+                                 "the transposed short-wide shapes"}, "reference_api_4096x16x1024": ref_api,
+          "shapes": rows})
 
 
 def bench_sweep(args, dev):
@@ -750,6 +770,7 @@ def compact_configs(args, dev):
     log(f"cfg2: n=128 NN {rows[-1]['NN']['us']} us, violations {out['cfg2_sweep']['violations']}")
     mp = bench_mp(args, dev)
     out["cfg3_mp"] = {"gflops_bf16x3": mp["value"], "gflops_fp32": mp["fp32_gflops"],
+                      "reference_api_4096x16x1024": mp["reference_api_4096x16x1024"],
                       "columns": "M,N,K,tb,fp32_us,fp32_roof_frac,fp32_max_rel,bf16x3_us,bf16x3_roof_frac,"
                                  "bf16x3_max_rel",
                       "max_rel_fp32": max(r["fp32"]["max_rel"] for r in mp["shapes"]),

@@ -5,8 +5,9 @@
  * Drop-in for the reference header NN_LIB/LibShalom.h with these changes:
  *   - every symbol is extern "C" (the reference's are C++-mangled because
  *     its header is C++, NN_LIB/LibShalom.h:5-7);
- *   - A, B and C are DEVICE pointers (hipMalloc / torch tensors / managed
- *     memory) on the current HIP device;
+ *   - A, B and C may be device pointers (hipMalloc / torch tensors / managed
+ *     memory) on the current HIP device, or host memory as in the
+ *     reference (the reference-API calls stage it; see below);
  *   - transa is honoured (the reference ignores it, NN_SMM.c:8,
  *     SMM_thread.c:81) and dgemm supports Trans (a no-op in the reference,
  *     NN_SMM.c:33-37);
@@ -65,8 +66,10 @@ typedef void *LibShalom_stream_t;
  * leave C untouched and are reported by LibShalom_get_last_error().
  * Operands may be device pointers or, as in the reference, host memory: host
  * operands are staged through a device buffer (inputs copied in, C copied
- * back), so the reference's harnesses run unchanged.  The other entry points
- * take device-accessible pointers only.
+ * back, through pinned bounce buffers with the copies overlapped, one
+ * synchronisation per call), so the reference's harnesses run unchanged.
+ * Calls from several host threads are safe.  The other entry points take
+ * device-accessible pointers only.
  * ------------------------------------------------------------------------ */
 
 /* Replaces LibShalom_sgemm (NN_LIB/LibShalom.h:22, NN_SMM.c:5-27). */
@@ -112,9 +115,18 @@ int LibShalom_dgemm_ex(int transa, int transb, double *C, const double *A, const
  *   C = alpha * op(A) op(B) + beta * C, row-major,
  * A is M x K (lda >= K) or, with transa, K x M (lda >= M); B is K x N
  * (ldb >= N) or, with transb, N x K (ldb >= K); C is M x N (ldc >= N).  C is
- * not read when beta is 0 (NaN or Inf in C do not propagate then).  Leading
+ * not read when beta is 0 (NaN or Inf in C do not propagate then), and A
+ * and B are not read when alpha is 0 (they may then be NULL).  Leading
  * dimensions below the minimum return LIBSHALOM_INVALID_VALUE; at or above
- * 2^21 elements, LIBSHALOM_NOT_SUPPORTED.  Asynchronous on `stream`. */
+ * 2^21 elements, LIBSHALOM_NOT_SUPPORTED.  Asynchronous on `stream`.
+ *
+ * Streams and capture (every asynchronous entry point): device buffers the
+ * library keeps per stream (split-K partial tiles and arrival counters, a
+ * convolution's flipped filter) grow on demand and an outgrown buffer is
+ * kept, not freed, until the process ends, so graphs that recorded it stay
+ * valid.  A call under stream capture that would need a buffer to grow
+ * returns LIBSHALOM_NOT_SUPPORTED; one eager call of that size on the
+ * stream beforehand sizes it. */
 int LibShalom_sgemm_general(int transa, int transb, long M, long N, long K, float alpha, const float *A,
                             long lda, const float *B, long ldb, float beta, float *C, long ldc,
                             int precision, LibShalom_stream_t stream);

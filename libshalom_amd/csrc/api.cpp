@@ -157,37 +157,67 @@ bool device_accessible(const void *p)
 }
 
 // ---------------------------------------------------------------------------
-// Per (device, stream) workspace for split-K partial sums.
+// Per (device, stream) device buffers: the workspace (a convolution's
+// flipped filter) and the fixup space of the in-launch reductions.
+//
+// A buffer that has to grow is replaced, and the old one is kept, not
+// freed, until the process ends: a kernel another host thread is about to
+// enqueue with it (its caller took the pointer before the growth) and a
+// captured graph that recorded it both stay valid.  Growth at least doubles,
+// so the kept buffers add up to less than the live one.  A stream in graph
+// capture cannot grow its buffers (allocation and the zeroing of a new
+// fixup space would happen outside the graph): such a call returns
+// LIBSHALOM_NOT_SUPPORTED; an eager call of the same size on that stream
+// beforehand makes it possible.
 // ---------------------------------------------------------------------------
-struct Workspace {
-    void *ptr = nullptr;
+struct GrowBuf {
+    char *ptr = nullptr;
     size_t size = 0;
+    std::vector<char *> retired;
 };
+
+bool stream_capturing(hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    return cs != hipStreamCaptureStatusNone;
+}
+
+// Makes b hold at least `bytes` (at least `floor`), retiring the old buffer.
+// *grown tells whether a new buffer was allocated.
+int grow_buf(GrowBuf &b, hipStream_t s, size_t bytes, size_t floor, bool *grown)
+{
+    *grown = false;
+    if (b.size >= bytes) return LIBSHALOM_SUCCESS;
+    if (stream_capturing(s)) return LIBSHALOM_NOT_SUPPORTED;
+    const size_t sz = std::max({bytes, floor, 2 * b.size});
+    char *p = nullptr;
+    if (hipMalloc((void **)&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        return LIBSHALOM_ALLOC_FAILED;
+    }
+    if (b.ptr) b.retired.push_back(b.ptr);
+    b.ptr = p;
+    b.size = sz;
+    *grown = true;
+    return LIBSHALOM_SUCCESS;
+}
+
 std::mutex g_ws_mu;
-std::map<std::pair<int, hipStream_t>, Workspace> g_ws;
+std::map<std::pair<int, hipStream_t>, GrowBuf> g_ws;
 
 int workspace(int dev, hipStream_t s, size_t bytes, void **out)
 {
     if (const char *f = std::getenv("LIBSHALOM_FAIL_WORKSPACE"))
         if (f[0] == '1') return LIBSHALOM_ALLOC_FAILED;  // failure injection for tests
     std::lock_guard<std::mutex> lk(g_ws_mu);
-    Workspace &w = g_ws[{dev, s}];
-    if (w.size < bytes) {
-        if (w.ptr) {
-            // earlier kernels on this stream may still read the old buffer
-            if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
-            (void)hipFree(w.ptr);
-            w.ptr = nullptr;
-            w.size = 0;
-        }
-        size_t sz = std::max(bytes, (size_t)1 << 20);
-        if (hipMalloc(&w.ptr, sz) != hipSuccess) {
-            (void)hipGetLastError();
-            w.ptr = nullptr;
-            return LIBSHALOM_ALLOC_FAILED;
-        }
-        w.size = sz;
-    }
+    GrowBuf &w = g_ws[{dev, s}];
+    bool grown;
+    const int st = grow_buf(w, s, bytes, (size_t)1 << 20, &grown);
+    if (st) return st;
     *out = w.ptr;
     return LIBSHALOM_SUCCESS;
 }
@@ -196,15 +226,10 @@ int workspace(int dev, hipStream_t s, size_t bytes, void **out)
 // In-launch reductions (split-K of single GEMMs, sliced backward / strided
 // problems).
 // ---------------------------------------------------------------------------
-// Per (device, stream) memory for the split-reduction fixup: arrival
-// counters (zero between launches; every launch leaves them zero) and the
-// partial-tile slots.
-struct FixupSpace {
-    char *ptr = nullptr;
-    size_t size = 0;
-};
+// The fixup space of a (device, stream): arrival counters (zero between
+// launches; every launch leaves them zero) and the partial-tile slots.
 std::mutex g_fx_mu;
-std::map<std::pair<int, hipStream_t>, FixupSpace> g_fx;
+std::map<std::pair<int, hipStream_t>, GrowBuf> g_fx;
 constexpr size_t kCounterBytes = 1 << 20;  // counters of 128 K tiles (two ints each)
 // Partial tiles a single GEMM's split-K may use (the planner picks fewer K
 // slices when more would be needed).
@@ -220,23 +245,13 @@ int fixup_space(int dev, hipStream_t s, size_t slot_bytes, int **counters, float
     if (const char *f = std::getenv("LIBSHALOM_FAIL_WORKSPACE"))
         if (f[0] == '1') return LIBSHALOM_ALLOC_FAILED;  // failure injection for tests
     std::lock_guard<std::mutex> lk(g_fx_mu);
-    FixupSpace &w = g_fx[{dev, s}];
-    const size_t need = kCounterBytes + slot_bytes;
-    if (w.size < need) {
-        if (w.ptr) {
-            if (hipStreamSynchronize(s) != hipSuccess) return LIBSHALOM_HIP_ERROR;
-            (void)hipFree(w.ptr);
-            w.ptr = nullptr;
-            w.size = 0;
-        }
-        const size_t sz = std::max(need, (size_t)8 << 20);
-        if (hipMalloc((void **)&w.ptr, sz) != hipSuccess) {
-            (void)hipGetLastError();
-            w.ptr = nullptr;
-            return LIBSHALOM_ALLOC_FAILED;
-        }
-        w.size = sz;
-        // a fill kernel rather than a memset, so that a captured call records a kernel node
+    GrowBuf &w = g_fx[{dev, s}];
+    bool grown;
+    int st = grow_buf(w, s, kCounterBytes + slot_bytes, (size_t)8 << 20, &grown);
+    if (st) return st;
+    if (grown) {
+        // zeroed on the stream, ahead of any kernel that gets this pointer
+        // (it is handed out under the lock, after this launch)
         hipLaunchKernelGGL(k_zero_ints, dim3(256), dim3(256), 0, s, (int *)w.ptr, (int)(kCounterBytes / sizeof(int)));
         if (hipGetLastError() != hipSuccess) return LIBSHALOM_HIP_ERROR;
     }
@@ -814,16 +829,117 @@ int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch
 // ---------------------------------------------------------------------------
 // The reference-API calls (synchronous, null stream).  The reference takes
 // host pointers; here device pointers are used as they are, and operands in
-// host memory (pageable or pinned, which the GPU cannot address) are staged
-// through a per-device device buffer: inputs copied in, C copied back when
-// the product is done.  The staging buffer is held for the whole call.
+// host memory that the GPU cannot address (pageable) are staged through a
+// per-device device buffer: inputs copied in, C copied back when the
+// product is done.  The copies go through kBufs pinned bounce buffers of
+// kChunk bytes in rotation: the CPU fills (or empties) one while the DMA
+// engine moves others, and one synchronisation ends the call.  (A pageable
+// hipMemcpy per operand synchronises each time, and its device-to-host
+// direction runs at a third of the pinned rate.)  Small operands share one
+// chunk and travel with one copy; events are recorded only where a buffer
+// comes round again within the call.  The staging buffers are held for the
+// whole call.
 // ---------------------------------------------------------------------------
+constexpr size_t kChunk = (size_t)4 << 20;  // (4 x 4 MiB measured best of 1-8 MiB x 2-8 for 16 MiB)
+constexpr int kBufs = 4;
+
 struct Staging {
-    void *ptr = nullptr;
-    size_t size = 0;
+    char *dev = nullptr;  // device copy of the host operands
+    size_t dev_size = 0;
+    char *pin[kBufs] = {};          // pinned bounce buffers
+    hipEvent_t done[kBufs] = {};    // the last DMA recorded on pin[i]
 };
 std::mutex g_stage_mu;
 std::map<int, Staging> g_stage;
+
+int hip_ok(hipError_t e)
+{
+    if (e == hipSuccess) return LIBSHALOM_SUCCESS;
+    (void)hipGetLastError();
+    return LIBSHALOM_HIP_ERROR;
+}
+
+int staging_ready(Staging &sg, size_t dev_bytes)
+{
+    for (int i = 0; i < kBufs; ++i) {
+        if (sg.pin[i]) continue;
+        if (hipHostMalloc((void **)&sg.pin[i], kChunk, hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            sg.pin[i] = nullptr;
+            return LIBSHALOM_ALLOC_FAILED;
+        }
+        if (int st = hip_ok(hipEventCreateWithFlags(&sg.done[i], hipEventDisableTiming))) return st;
+    }
+    if (sg.dev_size < dev_bytes) {
+        if (sg.dev) (void)hipFree(sg.dev);  // the calls are synchronous: nothing uses it now
+        sg.dev = nullptr;
+        sg.dev_size = 0;
+        if (hipMalloc((void **)&sg.dev, dev_bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            sg.dev = nullptr;
+            return LIBSHALOM_ALLOC_FAILED;
+        }
+        sg.dev_size = dev_bytes;
+    }
+    return LIBSHALOM_SUCCESS;
+}
+
+// Host -> device: the pieces (host source, bytes) land one after another
+// from dst on.  A chunk may span pieces.  Every earlier call has finished
+// (the calls end with a synchronisation), so a buffer needs a wait only
+// when this transfer comes round to it again.
+int stage_in(Staging &sg, const std::pair<const void *, size_t> *pieces, int n, char *dst, hipStream_t s)
+{
+    size_t total = 0;
+    for (int i = 0; i < n; ++i) total += pieces[i].second;
+    const long chunks = (long)((total + kChunk - 1) / kChunk);
+    int p = 0;
+    size_t in_piece = 0;  // bytes of piece p already taken
+    for (long c = 0; c < chunks; ++c) {
+        const int b = (int)(c % kBufs);
+        if (c >= kBufs)
+            if (int st = hip_ok(hipEventSynchronize(sg.done[b]))) return st;  // chunk c - kBufs has left
+        size_t fill = 0;
+        while (p < n && fill < kChunk) {
+            const size_t take = std::min(pieces[p].second - in_piece, kChunk - fill);
+            std::memcpy(sg.pin[b] + fill, (const char *)pieces[p].first + in_piece, take);
+            fill += take;
+            in_piece += take;
+            if (in_piece == pieces[p].second) {
+                ++p;
+                in_piece = 0;
+            }
+        }
+        if (int st = hip_ok(hipMemcpyAsync(dst, sg.pin[b], fill, hipMemcpyHostToDevice, s))) return st;
+        if (c + kBufs < chunks)
+            if (int st = hip_ok(hipEventRecord(sg.done[b], s))) return st;
+        dst += fill;
+    }
+    return LIBSHALOM_SUCCESS;
+}
+
+// Device -> host after the work queued on s: up to kBufs chunks in flight
+// while the CPU copies out the oldest; the last one is waited for with a
+// stream synchronisation (cheaper than an event here).
+int stage_out(Staging &sg, void *dst, const char *src, size_t bytes, hipStream_t s)
+{
+    const long chunks = (long)((bytes + kChunk - 1) / kChunk);
+    long issued = 0;
+    for (long r = 0; r < chunks; ++r) {
+        for (; issued < chunks && issued < r + kBufs; ++issued) {  // chunk issued - kBufs is out
+            const int b = (int)(issued % kBufs);
+            const size_t off = (size_t)issued * kChunk, len = std::min(kChunk, bytes - off);
+            if (int st = hip_ok(hipMemcpyAsync(sg.pin[b], src + off, len, hipMemcpyDeviceToHost, s))) return st;
+            if (issued + 1 < chunks)
+                if (int st = hip_ok(hipEventRecord(sg.done[b], s))) return st;
+        }
+        const int b = (int)(r % kBufs);
+        if (int st = hip_ok(r + 1 < chunks ? hipEventSynchronize(sg.done[b]) : hipStreamSynchronize(s))) return st;
+        const size_t off = (size_t)r * kChunk;
+        std::memcpy((char *)dst + off, sg.pin[b], std::min(kChunk, bytes - off));
+    }
+    return LIBSHALOM_SUCCESS;
+}
 
 template <typename T>
 int reference_call(int prec, int ta, int tb, T *C, T *A, T *B, long M, long N, long K, bool mp)
@@ -842,36 +958,28 @@ int reference_call(int prec, int ta, int tb, T *C, T *A, T *B, long M, long N, l
         st = run_single(prec, ta, tb, C, A, B, M, N, K, mp, nullptr);
         return st ? st : sync_null();
     }
-    auto round = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t bA = hA ? round((size_t)M * K * sizeof(T)) : 0;
-    const size_t bB = hB ? round((size_t)K * N * sizeof(T)) : 0;
-    const size_t bC = hC ? round((size_t)M * N * sizeof(T)) : 0;
+    const size_t nA = hA ? (size_t)M * K * sizeof(T) : 0;
+    const size_t nB = hB ? (size_t)K * N * sizeof(T) : 0;
+    const size_t nC = hC ? (size_t)M * N * sizeof(T) : 0;
+    const size_t offC = (nA + nB + 255) & ~(size_t)255;  // A and B adjacent: one copy when small
     std::lock_guard<std::mutex> lk(g_stage_mu);
     Staging &sg = g_stage[dev];
-    if (sg.size < bA + bB + bC) {
-        if (sg.ptr) (void)hipFree(sg.ptr);  // hipFree waits for the device
-        sg.ptr = nullptr;
-        sg.size = 0;
-        if (hipMalloc(&sg.ptr, bA + bB + bC) != hipSuccess) {
-            (void)hipGetLastError();
-            sg.ptr = nullptr;
-            return LIBSHALOM_ALLOC_FAILED;
-        }
-        sg.size = bA + bB + bC;
-    }
-    char *base = (char *)sg.ptr;
-    T *dA = hA ? (T *)base : A;
-    T *dB = hB ? (T *)(base + bA) : B;
-    T *dC = hC ? (T *)(base + bA + bB) : C;
-    if (hA && hipMemcpy(dA, A, (size_t)M * K * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-        return (void)hipGetLastError(), LIBSHALOM_HIP_ERROR;
-    if (hB && hipMemcpy(dB, B, (size_t)K * N * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
-        return (void)hipGetLastError(), LIBSHALOM_HIP_ERROR;
-    st = run_single(prec, ta, tb, dC, dA, dB, M, N, K, mp, nullptr);
-    if (!st) st = sync_null();
-    if (!st && hC && hipMemcpy(C, dC, (size_t)M * N * sizeof(T), hipMemcpyDeviceToHost) != hipSuccess)
-        return (void)hipGetLastError(), LIBSHALOM_HIP_ERROR;
-    return st;
+    st = staging_ready(sg, offC + nC);
+    if (st) return st;
+    // A's bytes are a multiple of 4 (of 8 for FP64): B stays aligned after it
+    T *dA = hA ? (T *)sg.dev : A;
+    T *dB = hB ? (T *)(sg.dev + nA) : B;
+    T *dC = hC ? (T *)(sg.dev + offC) : C;
+    std::pair<const void *, size_t> in[2];
+    int n_in = 0;
+    if (hA) in[n_in++] = {A, nA};
+    if (hB) in[n_in++] = {B, nB};
+    st = stage_in(sg, in, n_in, sg.dev, nullptr);
+    if (!st) st = run_single(prec, ta, tb, dC, dA, dB, M, N, K, mp, nullptr);
+    if (!st && hC) st = stage_out(sg, C, (const char *)dC, nC, nullptr);
+    // also after an error: the next call finds the bounce buffers idle
+    const int st_sync = sync_null();
+    return st ? st : st_sync;
 }
 
 int sgemm_sync(int ta, int tb, float *C, float *A, float *B, long M, long N, long K, bool mp)
@@ -1002,9 +1110,14 @@ void tune_load_locked()
     if (!path || !path[0]) return;
     FILE *f = std::fopen(path, "r");
     if (!f) return;
+    // lines "prec ta tb class tile"; a line naming a tile without a
+    // persistent launcher for its mode and transposes (or out of range) is
+    // ignored, so that a plan never meets a missing kernel
+    const LaunchTable &lt = launch_table();
     int prec, ta, tb, cls, tile;
     while (std::fscanf(f, "%d %d %d %d %d", &prec, &ta, &tb, &cls, &tile) == 5)
-        if (prec >= 0 && prec < NPREC && (ta | tb) >= 0 && ta < 2 && tb < 2 && tile >= 0 && tile < NTILE)
+        if (prec >= 0 && prec < NPREC && (ta | tb) >= 0 && ta < 2 && tb < 2 && tile >= 0 && tile < NTILE &&
+            cls >= 0 && cls < 1000 && lt.persist[prec][ta][tb][tile] != nullptr)
             g_tune[tune_key(prec, ta, tb, cls)] = tile;
     std::fclose(f);
 }
@@ -1228,6 +1341,12 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     if (count < 0) return LIBSHALOM_INVALID_VALUE;
     if (count > 0 && (!C || !A || !B || !M || !N || !K)) return LIBSHALOM_INVALID_POINTER;
     if ((ta != 0 && ta != 1) || (tb != 0 && tb != 1)) return LIBSHALOM_INVALID_VALUE;
+    // every GEMM's arguments, before the device is touched
+    for (long g = 0; g < count; ++g) {
+        int st = check_dims(ta, tb, M[g], N[g], K[g]);
+        if (!st) st = check_ptrs(C[g], A[g], B[g], M[g], N[g], K[g], false);
+        if (st) return st;
+    }
     int dev;
     DeviceInfo info;
     int st = current_device(&dev, &info);
@@ -1241,10 +1360,6 @@ int build_plan(LibShalom_plan_t *out, int prec, int ta, int tb, void *const *C, 
     for (int t = 0; t < NTILE; ++t) buckets[t].tile = t;
     double flops = 0, bytes = 0;
     for (long g = 0; g < count; ++g) {
-        st = check_dims(ta, tb, M[g], N[g], K[g]);
-        if (st) return st;
-        st = check_ptrs(C[g], A[g], B[g], M[g], N[g], K[g], false);
-        if (st) return st;
         if (M[g] == 0 || N[g] == 0) continue;
         flops += 2.0 * M[g] * N[g] * K[g];
         bytes += (double)esize * (M[g] * K[g] + K[g] * N[g] + M[g] * N[g]);
@@ -1475,9 +1590,11 @@ int LibShalom_sgemm_general(int transa, int transb, long M, long N, long K, floa
     int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
     if (!st) st = check_dims(transa, transb, M, N, K);
     if (!st) st = check_lds(transa, transb, M, N, K, lda, ldb, ldc);
-    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    // alpha == 0: C = beta * C without reading A or B (as BLAS), which may then be null
+    const long Kr = alpha == 0 ? 0 : K;
+    if (!st) st = check_ptrs(C, A, B, M, N, Kr, false);
     if (!st)
-        st = run_general(kprec(precision), transa, transb, C, A, B, M, N, K, lda, ldb, ldc, alpha, beta, false,
+        st = run_general(kprec(precision), transa, transb, C, A, B, M, N, Kr, lda, ldb, ldc, alpha, beta, false,
                          (hipStream_t)stream);
     return set_status(st);
 }
@@ -1489,9 +1606,11 @@ int LibShalom_sgemm_general_noalloc(int transa, int transb, long M, long N, long
     int st = valid_precision(precision) ? LIBSHALOM_SUCCESS : LIBSHALOM_INVALID_VALUE;
     if (!st) st = check_dims(transa, transb, M, N, K);
     if (!st) st = check_lds(transa, transb, M, N, K, lda, ldb, ldc);
-    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    // alpha == 0: C = beta * C without reading A or B (as BLAS), which may then be null
+    const long Kr = alpha == 0 ? 0 : K;
+    if (!st) st = check_ptrs(C, A, B, M, N, Kr, false);
     if (!st)
-        st = run_general(kprec(precision), transa, transb, C, A, B, M, N, K, lda, ldb, ldc, alpha, beta, false,
+        st = run_general(kprec(precision), transa, transb, C, A, B, M, N, Kr, lda, ldb, ldc, alpha, beta, false,
                          (hipStream_t)stream, true);
     return set_status(st);
 }
@@ -1501,9 +1620,11 @@ int LibShalom_dgemm_general(int transa, int transb, long M, long N, long K, doub
 {
     int st = check_dims(transa, transb, M, N, K);
     if (!st) st = check_lds(transa, transb, M, N, K, lda, ldb, ldc);
-    if (!st) st = check_ptrs(C, A, B, M, N, K, false);
+    // alpha == 0: C = beta * C without reading A or B (as BLAS), which may then be null
+    const long Kr = alpha == 0 ? 0 : K;
+    if (!st) st = check_ptrs(C, A, B, M, N, Kr, false);
     if (!st)
-        st = run_general(P_FP64, transa, transb, C, A, B, M, N, K, lda, ldb, ldc, alpha, beta, false,
+        st = run_general(P_FP64, transa, transb, C, A, B, M, N, Kr, lda, ldb, ldc, alpha, beta, false,
                          (hipStream_t)stream);
     return set_status(st);
 }

@@ -116,6 +116,15 @@ def test_general_null_pointers_rejected():
     assert L.LibShalom_sgemm_general(0, 0, 4, 4, 4, 1.0, 0x1000, 4, 0x1000, 4, 0.0, 0x1000, 4, 9, None) == 1
 
 
+def test_general_alpha_zero_needs_no_operands():
+    """BLAS leaves A and B unreferenced when alpha is 0: null A / B pass the
+    argument checks (and the call then stops at the missing device)."""
+    L = ls.lib()
+    assert L.LibShalom_sgemm_general(0, 0, 4, 4, 4, 0.0, None, 4, None, 4, 1.0, 0x1000, 4, ls.FP32, None) == 6
+    assert L.LibShalom_dgemm_general(0, 0, 4, 4, 4, 0.0, None, 4, None, 4, 0.0, 0x1000, 4, None) == 6
+    assert L.LibShalom_sgemm_general(0, 0, 4, 4, 4, 0.0, None, 4, None, 4, 1.0, None, 4, ls.FP32, None) == 2
+
+
 def test_backward_empty_batch_is_a_no_op():
     L = ls.lib()
     assert L.LibShalom_sgemm_backward_strided(0, 0, None, None, None, None, None, 4, 4, 4, 0, 0, 0, 0,
@@ -306,6 +315,61 @@ def test_tune_file_is_loaded(tmp_path):
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True,
                          cwd=os.path.dirname(os.path.dirname(ls.LIB_PATH)))
     assert out.stdout.split() == ["(128,", "128)", "(32,", "32)"]
+
+
+def test_tune_file_ignores_tiles_without_a_persistent_kernel(tmp_path):
+    """Lines naming a tile that has no persistent kernel for the mode (the
+    rectangular tiles), an out-of-range class or mode are ignored; the valid
+    line still applies."""
+    import subprocess
+    import sys
+    cls = (1 * 7 + 1) * 5 + 1  # (20, 20, 20)
+    path = tmp_path / "tune.txt"
+    path.write_text(f"{ls.BF16X3} 0 0 {cls} 3\n"      # 128 x 32: no persistent kernel
+                    f"{ls.BF16X3} 0 1 {cls} 5\n"      # 32 x 128: none either
+                    f"{ls.FP32} 0 0 {cls} 4\n"        # 128 x 64: none
+                    f"{ls.FP32} 1 1 -3 2\n"           # bad class
+                    f"9 0 0 {cls} 2\n"                # bad mode
+                    f"{ls.FP32} 0 1 {cls} 1\n")       # valid: 64 x 64
+    code = ("import libshalom_amd as ls; print(ls.query_tuned_tile(ls.BF16X3, 0, 0, 20, 20, 20), "
+            "ls.query_tuned_tile(ls.BF16X3, 0, 1, 20, 20, 20), ls.query_tuned_tile(ls.FP32, 0, 0, 20, 20, 20), "
+            "ls.query_tuned_tile(ls.FP32, 0, 1, 20, 20, 20))")
+    env = dict(os.environ, LIBSHALOM_TUNE_FILE=str(path))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, check=True,
+                         cwd=os.path.dirname(os.path.dirname(ls.LIB_PATH)))
+    assert out.stdout.split() == ["(32,", "32)", "(32,", "32)", "(32,", "32)", "(64,", "64)"]
+
+
+def test_c_dropin_program_builds_and_links():
+    """The C drop-in caller compiles as C99 against the header (-Wall -Wextra
+    -Werror) and links with -lsmm; without a GPU its call reports
+    LIBSHALOM_NO_DEVICE as its exit status."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(ls.LIB_PATH))
+    cdir = os.path.join(root, "tests", "c")
+    subprocess.run(["make", "-B", "-C", cdir], check=True, capture_output=True)
+    r = subprocess.run([os.path.join(cdir, "dropin_hello"), os.devnull], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode in (0, 6), (r.returncode, r.stdout, r.stderr)
+    assert "LibShalom_sgemm 80x80x80" in r.stdout
+
+
+def test_host_layer_under_address_and_ub_sanitizers():
+    """The host layer (api.cpp) rebuilt with -fsanitize=address,undefined
+    (host code only) and driven from four threads through its argument
+    checks, planner, tune-table lookups, plan builder and status bookkeeping
+    (tests/c/host_asan.cpp): no sanitizer report, every status as expected."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(ls.LIB_PATH))
+    cdir = os.path.join(root, "tests", "c")
+    subprocess.run(["make", "-C", cdir, "host_asan"], check=True, capture_output=True, timeout=600)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=1:abort_on_error=0:halt_on_error=1",
+               UBSAN_OPTIONS="print_stacktrace=1:halt_on_error=1")
+    r = subprocess.run([os.path.join(cdir, "host_asan")], capture_output=True, text=True, timeout=300, env=env,
+                       cwd=cdir)
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-4000:])
+    assert "host_asan: 0 failures" in r.stdout
+    assert "ERROR: AddressSanitizer" not in r.stderr and "runtime error" not in r.stderr
 
 
 # (batch, C, H, W, Cout, R, S, sh, sw, ph, pw, dh, dw, precision) -> status
