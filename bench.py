@@ -60,6 +60,8 @@ def main():
                     choices=["batched", "conv", "mp", "sweep", "hello", "dgemm_batched"])
     ap.add_argument("--conv-layers", default="",
                     help="conv workload: comma-separated indices of the layers to run (default all four)")
+    ap.add_argument("--mp-shapes", default="",
+                    help="mp workload: comma-separated MxNxK shapes of the grid to run (default all)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="issue one device-to-device copy of this many MiB first (untimed), a known-size "
                          "kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -76,9 +78,6 @@ def main():
     from libshalom_amd.workloads import uniform_shapes
 
     dev = torch.device("cuda:0")
-    if args.workload != "batched":
-        emit(OTHER[args.workload](args, dev))
-        return
     if args.calib_copy_mib > 0:
         n = args.calib_copy_mib * (1 << 20) // 4
         src = torch.ones(n, device=dev)
@@ -87,6 +86,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    if args.workload != "batched":
+        emit(OTHER[args.workload](args, dev))
+        return
     shapes = uniform_shapes(args.count, 1, 128, seed=0)
     Ms = np.array([s[0] for s in shapes], dtype=np.int64)
     Ns = np.array([s[1] for s in shapes], dtype=np.int64)
@@ -538,6 +540,9 @@ def bench_mp(args, dev):
     Cbuf = torch.empty(65536 * 1024, device=dev)
     shapes = [(M, N, K, "tall") for (M, N, K) in TALL_SKINNY]
     shapes += [(N, M, K, "wide") for (M, N, K) in TALL_SKINNY]  # short-wide: M small, N large
+    if getattr(args, "mp_shapes", ""):
+        keep = {tuple(int(v) for v in x.split("x")) for x in args.mp_shapes.split(",")}
+        shapes = [s for s in shapes if s[:3] in keep]
     rows = []
     rng = torch.Generator().manual_seed(1)
     for (M, N, K, kind) in shapes:
