@@ -329,6 +329,44 @@ struct TileCfg {
     static constexpr int NW = NT / 64;
     static constexpr int GN = BM == BN ? (NW >= 2 ? 2 : 1) : (BM > BN ? 1 : NW), GM = NW / GN;
     static constexpr int WM = BM / 16 / GM, WN = BN / 16 / GN;
+    // Which sub-tiles a wave owns.  FP32 / FP64 (INTERLEAVE): wave (g, h) of
+    // the grid owns the 16-row blocks g, g + GM, ... and the 16-column blocks
+    // h, h + GN, ...; a GEMM smaller than the tile covers its top-left
+    // corner, and interleaved blocks spread that corner over the waves, whose
+    // MFMAs skip the sub-tiles outside the GEMM.  The split modes, which
+    // multiply every sub-tile of a wave holding part of the GEMM, keep
+    // contiguous blocks (rows g*WM .. g*WM + WM - 1), so fewer waves hold
+    // one.  In an eight-wave workgroup waves w and w + 4 run on one SIMD
+    // (the hardware's placement, read from HW_ID on gfx950), so wave w takes
+    // grid place g = w / 2, h = (w & 1) ^ (w / 4): each SIMD then holds the
+    // blocks (g, h) and (g + 2, 1 - h), and the corner of a GEMM splits
+    // evenly over the four SIMDs (on the 50k U{1..128} batch the busiest
+    // SIMD's share of the 128 x 128 tiles' MFMAs falls from 42% to 28%).
+    static constexpr bool INTERLEAVE = !SPLIT;
+    __host__ __device__ static constexpr int wave_g(int w)
+    {
+        return NW == 8 && INTERLEAVE ? w >> 1 : w / GN;
+    }
+    __host__ __device__ static constexpr int wave_h(int w)
+    {
+        return NW == 8 && INTERLEAVE ? (w & 1) ^ (w >> 2) : w % GN;
+    }
+    // 16-row (16-column) block of the wave's a-th sub-tile row (column).
+    __host__ __device__ static constexpr int row_block(int g, int a) { return INTERLEAVE ? g + a * GM : g * WM + a; }
+    __host__ __device__ static constexpr int col_block(int h, int b) { return INTERLEAVE ? h + b * GN : h * WN + b; }
+    // How many of the wave's sub-tile rows (columns) lie in the first mt
+    // (nt) blocks -- always a prefix of them.
+    __host__ __device__ static constexpr int rows_in(int g, int mt)
+    {
+        return INTERLEAVE ? (g < mt ? ((mt - g + GM - 1) / GM < WM ? (mt - g + GM - 1) / GM : WM) : 0)
+                          : (mt - g * WM <= 0 ? 0 : (mt - g * WM < WM ? mt - g * WM : WM));
+    }
+    __host__ __device__ static constexpr int cols_in(int h, int nt)
+    {
+        return INTERLEAVE ? (h < nt ? ((nt - h + GN - 1) / GN < WN ? (nt - h + GN - 1) / GN : WN) : 0)
+                          : (nt - h * WN <= 0 ? 0 : (nt - h * WN < WN ? nt - h * WN : WN));
+    }
+    static_assert(NW != 8 || (GM == 4 && GN == 2), "the eight-wave placement assumes a 4 x 2 grid");
     // Waves per SIMD the persistent kernel is compiled for: four for the
     // eight-wave tiles of FP32 / BF16 / BF16X3 (two workgroups per CU, at
     // most 128 registers), two for the other 128 x 128 tiles.  (The
@@ -817,25 +855,26 @@ struct Tile {
     // single-GEMM tile kernel sets it; elsewhere it stays a constant false.
     bool scale = false;
     T alpha = T(1), beta = T(0);
-    int wr0, wc0;  // this wave's block (WM x WN sub-tiles of 16x16) in 16-row/col units
-    int wm, wn;    // sub-tiles of the block that reach into the job
-    bool live;     // the block holds part of the job's C tile
+    int wr0, wc0;  // this wave's place (g, h) in the wave grid (Cfg::wave_g / wave_h)
+    int wm, wn;    // sub-tile rows / columns of the wave that reach into the job (a prefix)
+    bool live;     // the wave holds part of the job's C tile
 
-    // The waves own the blocks of the GM x GN wave grid.  Every
-    // sub-tile of a live block is multiplied and stored whether or not it
-    // reaches into the job: rows and columns past the job are zeros in LDS,
-    // and their stores are masked by the buffer range.  Issuing the MFMAs
-    // without per-sub-tile branches lets the LDS reads run ahead of them; a
-    // block wholly outside the job skips its work with one uniform branch.
-    // The FP32 and FP64 products, whose MFMAs cost 4-8x more per element than
-    // the BF16 ones, skip the sub-tiles outside the job (uniform branches).
+    // The waves own WM x WN sub-tiles each (Cfg::row_block / col_block).  In
+    // the split modes every sub-tile of a live wave is multiplied and stored
+    // whether or not it reaches into the job: rows and columns past the job
+    // are zeros in LDS, and their stores are masked by the buffer range.
+    // Issuing the MFMAs without per-sub-tile branches lets the LDS reads run
+    // ahead of them; a wave wholly outside the job skips its work with one
+    // uniform branch.  The FP32 and FP64 products, whose MFMAs cost 4-8x
+    // more per element than the BF16 ones, skip the sub-tiles outside the
+    // job (uniform branches).
     __device__ __forceinline__ void layout(int mt, int nt, int wave)
     {
-        wr0 = (wave / Cfg::GN) * WM;
-        wc0 = (wave % Cfg::GN) * WN;
-        live = uniform(wr0 < mt && wc0 < nt) != 0;
-        wm = min(WM, mt - wr0);
-        wn = min(WN, nt - wc0);
+        wr0 = Cfg::wave_g(wave);
+        wc0 = Cfg::wave_h(wave);
+        wm = Cfg::rows_in(wr0, mt);
+        wn = Cfg::cols_in(wc0, nt);
+        live = uniform(wm > 0 && wn > 0) != 0;
     }
 
     __device__ __forceinline__ void zero()
@@ -875,7 +914,8 @@ struct Tile {
             for (int a = 0; a < WM; ++a)
 #pragma unroll
                 for (int pl = 0; pl < NP; ++pl)
-                    af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + ((wr0 + a) * 16 + r) * LDKH + swz_k(r, q * 8));
+                    af[a][pl] = *(const bf16x8 *)(As + pl * BM * LDKH + (Cfg::row_block(wr0, a) * 16 + r) * LDKH +
+                                                  swz_k(r, q * 8));
             // partial products (A piece, B piece), smallest first.  Two
             // pieces: lo*hi, hi*lo, hi*hi.  Three pieces (exact for every
             // FP32 value): all but lo2*lo2 (~2^-32 |a||b|) -- the two
@@ -889,7 +929,8 @@ struct Tile {
                 bf16x8 bfr[NP];
 #pragma unroll
                 for (int pl = 0; pl < NP; ++pl)
-                    bfr[pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + ((wc0 + b) * 16 + r) * LDKH + swz_k(r, q * 8));
+                    bfr[pl] = *(const bf16x8 *)(Bs + pl * BN * LDKH + (Cfg::col_block(wc0, b) * 16 + r) * LDKH +
+                                                 swz_k(r, q * 8));
 #pragma unroll
                 for (int p = 0; p < NPROD; ++p) {
                     const int ia = NP == 1 ? 0 : PA[p], ib = NP == 1 ? 0 : PB[p];
@@ -908,13 +949,17 @@ struct Tile {
         const T *Bs = (const T *)(buf + Cfg::A_BYTES);
         T af[WM][NS];
 #pragma unroll
-        for (int a = 0; a < WM; ++a) read_frag<typename Cfg::PA>(As, wr0 + a, r, q, af[a]);
+        for (int a = 0; a < WM; ++a) read_frag<typename Cfg::PA>(As, Cfg::row_block(wr0, a), r, q, af[a]);
+        // a live wave holds at least one sub-tile row and column (telling
+        // the compiler so keeps the partial-row path to the rows that can
+        // be missing)
+        __builtin_assume(wm >= 1 && wm <= WM && wn >= 1 && wn <= WN);
         const bool all_rows = wm == WM;
 #pragma unroll
         for (int b = 0; b < WN; ++b) {
             if (b >= wn) break;
             T bf[NS];
-            read_frag<typename Cfg::PB>(Bs, wc0 + b, r, q, bf);
+            read_frag<typename Cfg::PB>(Bs, Cfg::col_block(wc0, b), r, q, bf);
             if (all_rows) {
 #pragma unroll
                 for (int s = 0; s < NS; ++s)
@@ -1251,7 +1296,7 @@ struct Tile {
         // the wave's block (as layout() places it), derived afresh so that
         // nothing stays live across the stream for this rare path
         const int wave = uniform(threadIdx.x >> 6);
-        const int wr0 = (wave / Cfg::GN) * WM, wc0 = (wave % Cfg::GN) * WN;
+        const int wr0 = Cfg::wave_g(wave), wc0 = Cfg::wave_h(wave);
         for (int job = lo; job <= hi; job += stride) {
             const GemmDesc dj = src.get(job);
             if (dj.flags & kChain) continue;  // stored with the chain's last job
@@ -1259,7 +1304,8 @@ struct Tile {
 #pragma clang loop unroll(disable)
             for (int e = 0; e < WM * WN * 4; ++e) {
                 const int ab = e >> 2, a = ab / WN, b = ab - a * WN;
-                const int row = (wr0 + a) * 16 + acc_row(q, e & 3), col = (wc0 + b) * 16 + r;
+                const int row = Cfg::row_block(wr0, a) * 16 + acc_row(q, e & 3);
+                const int col = Cfg::col_block(wc0, b) * 16 + r;
                 if (row >= dj.M || col >= dj.N) continue;
                 float *o = (float *)dj.C + (long long)row * dj.ldc + cm(col);
                 if (!__builtin_isfinite(*o)) *o = dot_fp32(src, j0, job, stride, row, col);
@@ -1291,7 +1337,8 @@ struct Tile {
             const int e = __builtin_ctzll(bad);
             bad &= bad - 1;
             const int ab = e >> 2, a = ab / WN, b = ab - a * WN;
-            const int row = (wr0 + a) * 16 + acc_row(q, e & 3), col = (wc0 + b) * 16 + r;
+            const int row = Cfg::row_block(wr0, a) * 16 + acc_row(q, e & 3);
+            const int col = Cfg::col_block(wc0, b) * 16 + r;
             if (row >= dj.M || col >= dj.N) continue;
             float v = dot_fp32(src, j0, job, stride, row, col);
             float *o = (float *)dj.C + (long long)row * dj.ldc + cm(col);
@@ -1372,8 +1419,8 @@ struct Tile {
         for (int a = 0; a < WM; ++a)
 #pragma unroll
             for (int b = 0; b < WN; ++b) {
-                const int col = (wc0 + b) * 16 + r;
-                const int row = (wr0 + a) * 16 + acc_row(q, 0);
+                const int col = Cfg::col_block(wc0, b) * 16 + r;
+                const int row = Cfg::row_block(wr0, a) * 16 + acc_row(q, 0);
                 const int off = (row * ldo + cm(col)) * ES_OUT | (col < N ? 0 : kOOB);
                 // element i's offset, out of range when `skip` leaves it out
                 auto at = [&](int i) { return (skip >> ((a * WN + b) * 4 + i)) & 1 ? kOOB : off + i * rstep; };
@@ -1381,7 +1428,7 @@ struct Tile {
                     // rows past M masked one by one (no alpha / beta here)
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
-                        const int ri = (wr0 + a) * 16 + acc_row(q, i);
+                        const int ri = Cfg::row_block(wr0, a) * 16 + acc_row(q, i);
                         buf_store1<OutT>(rs, ri < M ? at(i) : kOOB, (OutT)acc[a][b][i]);
                     }
                 } else if (scale) {

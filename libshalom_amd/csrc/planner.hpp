@@ -59,52 +59,43 @@ inline double subtile_chunk_cycles(int prec)
 
 inline long cdiv(long a, long b) { return (a + b - 1) / b; }
 
-// The wave grid of each tile shape (TileCfg): nw waves in a gm x gn grid,
-// each owning wm x wn sub-tiles of 16 x 16.
-struct WaveGrid {
-    int nw, gm, gn, wm, wn;
-};
-template <int BM, int BN>
-constexpr WaveGrid wave_grid_of()
-{
-    typedef TileCfg<P_FP32, 0, 0, BM, BN> C;
-    return WaveGrid{C::NW, C::GM, C::GN, C::WM, C::WN};
-}
-inline WaveGrid wave_grid(int tile)
-{
-    static const WaveGrid g[NTILE] = {wave_grid_of<32, 32>(),  wave_grid_of<64, 64>(),  wave_grid_of<128, 128>(),
-                                      wave_grid_of<128, 32>(), wave_grid_of<128, 64>(), wave_grid_of<32, 128>()};
-    return g[tile];
-}
-
-// Waves of a tile that hold part of an M x N GEMM (its first tile).
-inline int live_waves(int tile, long M, long N)
-{
-    const WaveGrid g = wave_grid(tile);
-    const TileShape ts = kTileShapes[tile];
-    const long mt = cdiv(std::min<long>(ts.bm, M), 16), nt = cdiv(std::min<long>(ts.bn, N), 16);
-    int live = 0;
-    for (int w = 0; w < g.nw; ++w) live += (w / g.gn) * g.wm < mt && (w % g.gn) * g.wn < nt;
-    return live;
-}
-
 // MFMA cycles one K chunk of a tile costs on its busiest SIMD (waves w and
-// w + 4 of an eight-wave tile share SIMD w % 4).  FP32 / FP64 waves multiply
-// the sub-tiles that reach into the GEMM; the split modes every sub-tile of
-// a wave holding part of it.
+// w + 4 of an eight-wave tile share SIMD w % 4), for a GEMM covering mt x nt
+// 16 x 16 blocks of it.  FP32 / FP64 waves multiply the sub-tiles that reach
+// into the GEMM; the split modes every sub-tile of a wave holding part of
+// it.  The wave placement is the kernels' own (TileCfg::wave_g, rows_in, ...).
+template <class C>
+inline double busiest_simd(long M, long N)
+{
+    const int mt = (int)cdiv(std::min<long>(C::BM, M), 16), nt = (int)cdiv(std::min<long>(C::BN, N), 16);
+    double simd[4] = {0, 0, 0, 0};
+    for (int w = 0; w < C::NW; ++w) {
+        const long rows = C::rows_in(C::wave_g(w), mt), cols = C::cols_in(C::wave_h(w), nt);
+        simd[w % 4] += rows * cols == 0 ? 0.0 : C::SPLIT ? (double)C::WM * C::WN : (double)rows * cols;
+    }
+    return *std::max_element(simd, simd + 4);
+}
+
+// The same for any tile shape; the placement depends only on whether the
+// mode splits its inputs (PREC stands for its class).
+template <int PREC>
+inline double busiest_simd_tile(int tile, long M, long N)
+{
+    switch (tile) {
+    case T32x32: return busiest_simd<TileCfg<PREC, 0, 0, 32, 32>>(M, N);
+    case T64x64: return busiest_simd<TileCfg<PREC, 0, 0, 64, 64>>(M, N);
+    case T128x128: return busiest_simd<TileCfg<PREC, 0, 0, 128, 128>>(M, N);
+    case T128x32: return busiest_simd<TileCfg<PREC, 0, 0, 128, 32>>(M, N);
+    case T128x64: return busiest_simd<TileCfg<PREC, 0, 0, 128, 64>>(M, N);
+    default: return busiest_simd<TileCfg<PREC, 0, 0, 32, 128>>(M, N);
+    }
+}
+
 inline double tile_chunk_mfma(int prec, int tile, long M, long N)
 {
-    const WaveGrid g = wave_grid(tile);
-    const TileShape ts = kTileShapes[tile];
-    const long mt = cdiv(std::min<long>(ts.bm, M), 16), nt = cdiv(std::min<long>(ts.bn, N), 16);
     const bool split = prec == P_BF16X3 || prec == P_BF16 || prec == P_BF16X6;
-    double simd[4] = {0, 0, 0, 0};
-    for (int w = 0; w < g.nw; ++w) {
-        const long rows = std::max(0L, std::min<long>(g.wm, mt - (w / g.gn) * g.wm));
-        const long cols = std::max(0L, std::min<long>(g.wn, nt - (w % g.gn) * g.wn));
-        simd[w % 4] += rows * cols == 0 ? 0.0 : split ? (double)g.wm * g.wn : (double)rows * cols;
-    }
-    return *std::max_element(simd, simd + 4) * subtile_chunk_cycles(prec);
+    const double busiest = split ? busiest_simd_tile<P_BF16X3>(tile, M, N) : busiest_simd_tile<P_FP32>(tile, M, N);
+    return busiest * subtile_chunk_cycles(prec);
 }
 
 // Estimated cycles for one GEMM with a given tile and split: rounds of

@@ -235,21 +235,26 @@ def test_planner_fills_the_chip():
 
 
 # The wave grid of each tile (TileCfg in gemm_core.hpp): 128 x 128 tiles run
-# eight waves in a 4 x 2 grid of 32 x 64 blocks; the other square tiles four
-# waves in a 2 x 2 grid; the tall tiles (128 x 32, 128 x 64) split their rows
-# over four waves, the wide one (32 x 128) its columns.
-WAVE_BLOCK = {(32, 32): (16, 16), (64, 64): (32, 32), (128, 128): (32, 64), (128, 32): (32, 32),
-              (128, 64): (32, 64), (32, 128): (32, 32)}
+# eight waves in a 4 x 2 grid; the other square tiles four waves in a 2 x 2
+# grid; the tall tiles (128 x 32, 128 x 64) split their rows over four waves,
+# the wide one (32 x 128) its columns.  In the split modes a wave owns a
+# contiguous block of 16 x 16 sub-tiles; in FP32 / FP64 its sub-tile rows and
+# columns are interleaved with the other waves' (row blocks g, g + GM, ...).
+WAVE_GRID = {(32, 32): (2, 2), (64, 64): (2, 2), (128, 128): (4, 2), (128, 32): (4, 1),
+             (128, 64): (4, 1), (32, 128): (1, 4)}
 
 
-def idle_fraction(tile, M, N):
+def idle_fraction(tile, M, N, prec):
     """Fraction of a tile's waves holding no part of an M x N GEMM (its first tile)."""
     bm, bn = tile
-    wr, wc = WAVE_BLOCK[tile]
-    waves_r, waves_c = bm // wr, bn // wc
-    live_r = min(waves_r, -(-min(bm, M) // wr))
-    live_c = min(waves_c, -(-min(bn, N) // wc))
-    return 1.0 - live_r * live_c / (waves_r * waves_c)
+    gm, gn = WAVE_GRID[tile]
+    mt, nt = -(-min(bm, M) // 16), -(-min(bn, N) // 16)
+    if prec in (ls.FP32, ls.FP64):
+        live_r, live_c = min(gm, mt), min(gn, nt)
+    else:
+        wm, wn = bm // 16 // gm, bn // 16 // gn
+        live_r, live_c = min(gm, -(-mt // wm)), min(gn, -(-nt // wn))
+    return 1.0 - live_r * live_c / (gm * gn)
 
 
 @pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3, ls.BF16X6, ls.FP64])
@@ -260,9 +265,9 @@ def test_planner_keeps_waves_busy_on_skinny_grid(prec):
     shapes = list(TALL_SKINNY) + [(N, M, K) for (M, N, K) in TALL_SKINNY]
     for (M, N, K) in shapes:
         tile = ls.query_plan(prec, M, N, K, num_cu=256, mp=True)["tile"]
-        if idle_fraction(tile, M, N) > 0.5:
-            others = [t for t in WAVE_BLOCK if not (prec == ls.FP64 and t == (128, 128))]
-            assert all(idle_fraction(t, M, N) > 0.5 for t in others), (M, N, K, tile)
+        if idle_fraction(tile, M, N, prec) > 0.5:
+            others = [t for t in WAVE_GRID if not (prec == ls.FP64 and t == (128, 128))]
+            assert all(idle_fraction(t, M, N, prec) > 0.5 for t in others), (M, N, K, tile)
 
 
 def test_planner_errors():
