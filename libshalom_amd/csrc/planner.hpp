@@ -141,6 +141,13 @@ inline SinglePlan plan_single(int prec, long M, long N, long K, bool /*mp*/, int
     const int esize = prec == P_FP64 ? 8 : 4;
     for (int t = 0; t < NTILE; ++t) {
         if (prec == P_FP64 && t == T128x128) continue;  // 128 accumulator registers: occupancy 1
+        // FP32: no 128 x 128 tile for a GEMM of at most 64 rows or columns.
+        // Half of its eight waves would hold nothing, yet run the chunk
+        // pipeline (loads, LDS writes, barriers) with the others.  The MFMA
+        // model alone ties it with the four-wave tall / wide tiles there
+        // (the busiest SIMD carries the same sub-tiles), and measured it is
+        // slower (4096 x 16 x 1024: 15.7 us against 10-12 us).
+        if (prec == P_FP32 && t == T128x128 && std::min(M, N) <= 64) continue;
         const TileShape ts = kTileShapes[t];
         const size_t ntiles = (size_t)cdiv(M, ts.bm) * (size_t)cdiv(N, ts.bn);
         for (int ks = 1; ks <= 64; ks *= 2) {
