@@ -529,10 +529,15 @@ void bwd_slice(BwdProblem &P, int T, long len)
 // a compute unit's share: a 128 x 128 workgroup fills its CU alone, and one
 // unit per CU there measured neutral to 1.4x faster than two on the conv
 // layers' backward launches; slices cost partial-tile traffic, which is
-// kept under a quarter of the operand traffic.  Returns
-// the tile size (0: nothing to launch, < 0: -status).
+// kept under a quarter of the operand traffic -- the larger of the
+// algorithmic bytes and what the units load (a shared operand is read by
+// every image's tiles: on the 512 x 49 x 4608 layer, batch 32, the units
+// load 13x the algorithmic bytes, and two slices per tile took its BF16X3
+// forward from 0.094 to 0.064 ms).  occ64: resident 64 x 64 workgroups per
+// CU the makespan model assumes (below).  Returns the tile size (0: nothing
+// to launch, < 0: -status).
 int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu, hipStream_t s,
-                  int big_quarters = 3)
+                  int big_quarters = 3, int occ64 = 2)
 {
     auto base_units = [&](int T) {
         long u = 0;
@@ -562,39 +567,50 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
         for (int it = 0; it < 40; ++it) {
             slot_bytes = counters = 0;
             units = 0;
+            double loads = 0;  // operand bytes the units load: each its L and R panels of the slice
             for (int i = 0; i < n; ++i) {
                 BwdProblem &P = *probs[i];
                 bwd_slice(P, T, len);
                 units += P.units;
+                loads += (double)P.units * 2.0 * T * (P.spi > 0 ? (double)P.slen : (double)P.gimg * P.red) * 4.0;
                 if (P.nslices > 1) {
                     slot_bytes += (size_t)P.units * T * T * sizeof(float);
                     counters += (size_t)(P.sO == 0 ? 1 : P.nimg) * P.ntiles;
                 }
             }
-            const bool traffic_ok = 2.0 * slot_bytes <= 0.25 * ops;
+            const bool traffic_ok = 2.0 * slot_bytes <= 0.25 * std::max(ops, loads);
             if (traffic_ok && slot_bytes <= ((size_t)256 << 20) && counters * 2 * sizeof(int) <= kCounterBytes) break;
             len *= 2;
         }
     };
-    // Of 1, 2 and 4 times that length, the one with the least estimated
-    // makespan: rounds of units over the resident workgroups (one per CU
-    // for 128 x 128 tiles, two for 64 x 64) times the longest unit's
-    // reduction steps; ties go to the longer slices (fewer partial tiles).
-    // (The 7x7 conv layer's backward, 300 units of 128 x 128 for 256 CUs at
-    // the base length, runs in one round of 248 at twice it: 0.21 -> 0.15 ms.)
-    const long resident = (long)num_cu * (T == 128 ? 1 : 2);
+    // Of 1, 2 and 4 times that length (and a quarter and a half of it when
+    // more than two 64 x 64 workgroups are resident), the one with the least
+    // estimated makespan: rounds of units over the resident workgroups (one
+    // per CU for 128 x 128 tiles, occ64 for 64 x 64) times the longest
+    // unit's reduction steps; ties go to the longer slices (fewer partial
+    // tiles).  (The 7x7 conv layer's backward, 300 units of 128 x 128 for
+    // 256 CUs at the base length, runs in one round of 248 at twice it:
+    // 0.21 -> 0.15 ms.  Single-problem launches -- the strided and the
+    // implicit-convolution forward -- plan with four resident workgroups:
+    // measured on the conv layers at batch 32, the 14x14 and 7x7 layers'
+    // forwards ran 8-18% faster (BF16X3 implicit 0.093 -> 0.076 ms on 7x7),
+    // the others unchanged; the two-problem backward launches ran up to 1.5x
+    // slower with it and keep two.)
+    const long resident = (long)num_cu * (T == 128 ? 1 : occ64);
     long best = len0;
     double best_cost = 0;
-    for (int m = 1; m <= 4; m *= 2) {
-        slice_all(len0 * m);
+    const int mlo = occ64 > 2 ? 1 : 4;  // quarters of len0: from a quarter with more resident units
+    for (int m = mlo; m <= 16; m *= 2) {
+        const long lenm = std::max(256L, ceil_div(len0 * m / 4, 32) * 32);
+        slice_all(lenm);
         double longest = 0;
         for (int i = 0; i < n; ++i) {
             const BwdProblem &P = *probs[i];
             if (P.units > 0) longest = std::max(longest, P.spi > 0 ? (double)P.slen : (double)P.gimg * P.red);
         }
         const double cost = (double)ceil_div(units, resident) * longest;
-        if (m == 1 || cost <= best_cost) {
-            best = len0 * m;
+        if (m == mlo || cost <= best_cost) {
+            best = lenm;
             best_cost = cost;
         }
     }
@@ -656,7 +672,8 @@ int strided_run(int prec, int ta, int tb, float *C, const float *A, const float 
     BwdProblem P;
     bwd_side(P, A, sA, (int)(ta ? M : K), B, sB, (int)(tb ? K : N), C, sC, (int)N, M, N, K, nimg);
     BwdProblem *probs[1] = {&P};
-    const int T = plan_problems(probs, 1, std::min(M, N) >= 128, dev, info.num_cu, s);
+    // one problem: its 64 x 64 workgroups are light enough for four per CU
+    const int T = plan_problems(probs, 1, std::min(M, N) >= 128, dev, info.num_cu, s, 3, 4);
     if (T <= 0) return -T;
     StridedLaunch fn = bwd_table().strided[prec][ta][tb][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
@@ -818,7 +835,7 @@ int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch
     bwd_side(Pb, Wt, 0, (int)K, X, C * H * W, 0, Y, Cout * PQ, (int)PQ, Cout, PQ, K, (int)batch);
     Pb.kalign = (int)(32 / std::gcd(32L, RS) * RS);
     BwdProblem *probs[1] = {&Pb};
-    const int T = plan_problems(probs, 1, std::min(Cout, (long)Pb.Cc) >= 128, dev, info.num_cu, s);
+    const int T = plan_problems(probs, 1, std::min(Cout, (long)Pb.Cc) >= 128, dev, info.num_cu, s, 3, 4);
     if (T <= 0) return -T;
     ConvLaunch fn = bwd_table().conv_fwd[prec][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
