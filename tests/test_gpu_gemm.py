@@ -197,12 +197,28 @@ def _sample_rows_check(dC, dA, dB, ta, tb, M, N, K, prec, rng, nrows=48):
     return check_f32(C, Asub, dB.cpu().numpy(), 0, tb, len(rows), N, K, prec)
 
 
+def _full_check_f64(dC, dA, dB, ta, tb, M, N, K, prec):
+    """Every element of C against a float64 product of the same operands
+    computed on the device (its own error, ~K 2^-53 |A||B|, is negligible
+    against the bound): |C - C64| <= (2 K u + extra) |A||B|."""
+    A = dA.view(*oracle.a_shape(ta, M, K)).double()
+    B = dB.view(*oracle.b_shape(tb, N, K)).double()
+    A = A.T.contiguous() if ta else A
+    B = B.T.contiguous() if tb else B
+    ref = A @ B
+    den = A.abs() @ B.abs()
+    err = (dC.view(M, N).double() - ref).abs()
+    bound = den * (2.0 * max(K, 1) * U32 + PREC_EXTRA[prec])
+    return int((~(err <= bound)).sum()), float((err / den.clamp_min(1e-300)).max())
+
+
 @pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
 @pytest.mark.parametrize("tb", [0, 1])
 def test_mp_tall_skinny(dev, prec, tb):
     """Irregular-shaped sgemm_mp: M in {4096, 16384, 65536} x N in {16, 32, 64}
-    x K in {64, 256, 1024}; NN and NT; rows sampled for the oracle, NaN
-    prefill checks every element is written."""
+    x K in {64, 256, 1024}; NN and NT; rows sampled for the oracle, every
+    element of C against a float64 product on the device, and NaN prefill
+    checks every element is written."""
     rng = np.random.default_rng(11 + tb)
     for M in (4096, 16384, 65536):
         for N in (16, 32, 64):
@@ -216,6 +232,8 @@ def test_mp_tall_skinny(dev, prec, tb):
                 assert bool(torch.isfinite(dC).all()), (M, N, K)
                 bad, st = _sample_rows_check(dC, dA, dB, 0, tb, M, N, K, prec, rng)
                 assert bad == 0, ((M, N, K), st)
+                bad, mx = _full_check_f64(dC, dA, dB, 0, tb, M, N, K, prec)
+                assert bad == 0, ((M, N, K), bad, mx)
 
 
 @pytest.mark.parametrize("ta,tb", TRANS)
@@ -230,6 +248,8 @@ def test_mp_short_wide_and_transposes(dev, ta, tb):
         assert bool(torch.isfinite(dC).all()), (M, N, K)
         bad, st = _sample_rows_check(dC, dA, dB, ta, tb, M, N, K, ls.FP32, rng)
         assert bad == 0, ((M, N, K), st)
+        bad, mx = _full_check_f64(dC, dA, dB, ta, tb, M, N, K, ls.FP32)
+        assert bad == 0, ((M, N, K), bad, mx)
 
 
 @pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X6])
