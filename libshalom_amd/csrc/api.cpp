@@ -23,6 +23,7 @@
 #include "backward.hpp"
 #include "kernels.hpp"
 #include "planner.hpp"
+#include "skinny.hpp"
 
 using namespace ls;
 
@@ -380,6 +381,45 @@ int noalloc_scratch(int dev, char **out)
     return LIBSHALOM_SUCCESS;
 }
 
+// The short-side kernel (skinny.hpp) for an FP32 GEMM whose C is tall (N <=
+// 64) or wide (M <= 64) with a long side L of at least 1024, when it beats
+// the tile kernels' split-K plans (measured on MI355X, sgemm_mp shapes, A/B
+// against the tile plans): K >= 512 (below that both are launch-bound); the
+// K x short-side work a CU's workgroups carry in series (rounds of L / 16
+// workgroups over the CUs, times K times the 16-column blocks) at most 4096,
+// and at most 2048 when the workgroups leave CUs idle (1024 x 64 x 4096:
+// 35 us against 19; 2048 x 32 x 2048: 14.5 against 12.0); a wide C in one
+// round (16 x 16384 x 1024 ran slower).  Wins: 4096 x 16 x 1024 11.6 -> 6.3
+// us, 2048 x 16 x 1024 9.6 -> 6.1 us, 16 x 4096 x 1024 11.3 -> 7.0 us,
+// 4096 x 16 x 4096 21.8 -> 17.1 us.  And the operands' byte offsets (past
+// their extents by at most a group of 16 indices) must stay in 31 bits.
+// Returns the kernel's column-block count index (0, 1, 2 for 16, 32, 64
+// short-side columns), or -1 for the tile kernels.  LIBSHALOM_SKINNY=0
+// turns it off (measurement).
+int skinny_choice(int prec, int ta, int tb, long M, long N, long K, long lda, long ldb, int num_cu)
+{
+    static const bool off = [] {
+        const char *e = std::getenv("LIBSHALOM_SKINNY");
+        return e && e[0] == '0';
+    }();
+    if (off || prec != P_FP32) return -1;
+    const bool wide = M <= 64 && N >= 1024, tall = N <= 64 && M >= 1024;
+    if (!wide && !tall) return -1;
+    const long L = wide ? N : M, S = wide ? M : N;
+    // X: op(A) (tall) or op(B)^T (wide); Y: the other, with their index counts
+    const bool xkc = wide ? tb != 0 : ta == 0, ykc = wide ? ta == 0 : tb != 0;
+    const long xld = wide ? ldb : lda, yld = wide ? lda : ldb;
+    auto fits = [&](bool kc, long count, long ld) {
+        const long span = kc ? (count + 16) * ld + K + 64 : (K + 64) * ld + count + 64;
+        return span * 4 < (1L << 31);
+    };
+    if (!fits(xkc, L, xld) || !fits(ykc, S + 48, yld)) return -1;
+    const int nc = S <= 16 ? 0 : S <= 32 ? 1 : 2;
+    const long nwg = (L + 15) / 16, rounds = (nwg + num_cu - 1) / num_cu, chain = K << nc;
+    if (K < 512 || rounds * chain > 4096 || (nwg < num_cu && chain > 2048) || (wide && rounds > 1)) return -1;
+    return nc;
+}
+
 // One GEMM on `stream`: plan tiles / split-K and launch.  With noalloc the
 // split-K counters and slots come from the static scratch array and the plan
 // is limited to what it holds; otherwise from the per-stream fixup space.
@@ -397,6 +437,14 @@ int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B,
     d.lda = (int)lda;
     d.ldb = (int)ldb;
     d.ldc = (int)ldc;
+    const int nc = skinny_choice(prec, ta, tb, M, N, K, lda, ldb, info.num_cu);
+    if (nc >= 0) {
+        SkinnyLaunch sk = skinny_table().fn[ta][tb][M <= 64][nc];
+        if (!sk) return LIBSHALOM_NOT_SUPPORTED;
+        const SkinnyArgs a{(const float *)d.A, (const float *)d.B, (float *)C, (int)M, (int)N, (int)K,
+                           (int)lda, (int)ldb, (int)ldc, (float)alpha, (float)beta};
+        return sk(a, stream) == hipSuccess ? LIBSHALOM_SUCCESS : LIBSHALOM_HIP_ERROR;
+    }
     // split-K needs the XCD placement of its slices (k_gemm_tiles); without
     // it the agent-scope fences make a split slower than none
     const size_t slot_limit = noalloc ? kNoAllocSlotBytes : kSplitKSlotLimit;
@@ -1869,6 +1917,17 @@ int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_
 {
     if (!valid_precision(precision) && precision != LIBSHALOM_FP64) return set_status(LIBSHALOM_INVALID_VALUE);
     if (M < 0 || N < 0 || K < 0 || num_cu <= 0) return set_status(LIBSHALOM_INVALID_VALUE);
+    // a short-sided FP32 GEMM (row-major, no transposes) runs on the
+    // short-side kernel: 16 long-side indices per workgroup, K whole
+    const int nc = skinny_choice(kprec(precision), 0, 0, M, N, K, K, N, num_cu);
+    if (nc >= 0) {
+        const int w = 16 << nc;
+        if (tile_m) *tile_m = M <= 64 ? w : 16;
+        if (tile_n) *tile_n = M <= 64 ? 16 : w;
+        if (ksplit) *ksplit = 1;
+        if (kper) *kper = (int)std::max<long>(K, 1);
+        return set_status(LIBSHALOM_SUCCESS);
+    }
     const SinglePlan p = plan_single(kprec(precision), M, N, K, mp != 0, num_cu);
     if (tile_m) *tile_m = kTileShapes[p.tile].bm;
     if (tile_n) *tile_n = kTileShapes[p.tile].bn;

@@ -88,7 +88,9 @@ def test_host_staging_in_chunks(dev, which):
 
 def test_reference_api_four_threads(dev):
     """Four threads on the null stream with different split-K sizes."""
-    shapes = [(2048, 16, 2048), (4096, 32, 1024), (512, 16, 4096), (1024, 64, 1536)]
+    # split-K shapes of different partial-tile sizes (M < 1024 keeps them off
+    # the FP32 short-side kernel, which needs no fixup space)
+    shapes = [(1000, 16, 2048), (900, 32, 1024), (512, 16, 4096), (768, 40, 3000)]
     assert len({ls.query_plan(ls.FP32, *s, mp=True)["ksplit"] * s[0] * s[1] for s in shapes}) > 1
     inputs, refs = [], []
     for t, (M, N, K) in enumerate(shapes):
@@ -125,7 +127,9 @@ def test_reference_api_four_threads(dev):
 
 
 def test_capture_refuses_growth_then_replays_after_growth(dev):
-    M, N, K = 2048, 16, 2048
+    # shapes on the tile kernels' split-K path (not short-sided: FP32 GEMMs
+    # with N or M <= 64 take the short-side kernel, which needs no space)
+    M, N, K = 128, 128, 8192
     assert ls.query_plan(ls.FP32, M, N, K, mp=True)["ksplit"] > 1
     rng = np.random.default_rng(3)
     A = rng.uniform(-1, 1, (M, K)).astype(np.float32)
@@ -151,7 +155,7 @@ def test_capture_refuses_growth_then_replays_after_growth(dev):
     with torch.cuda.graph(g, stream=s):
         ls.sgemm_mp_ex(0, 0, dC, dA, dB, M, N, K, ls.FP32, s)
     # a larger eager call on the same stream replaces (and keeps) the buffer
-    M2, N2, K2 = 8192, 64, 4096
+    M2, N2, K2 = 512, 512, 16384
     assert ls.query_plan(ls.FP32, M2, N2, K2, mp=True)["ksplit"] > 1
     dA2 = torch.rand(M2, K2, device=dev)
     dB2 = torch.rand(K2, N2, device=dev)

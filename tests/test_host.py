@@ -206,6 +206,9 @@ SHAPES = [(80, 80, 80), (8, 8, 8), (128, 128, 128), (4096, 16, 1024), (65536, 64
           (256, 2048, 5000), (31999, 1024, 88)]
 
 
+TILES = {(32, 32), (64, 64), (128, 128), (128, 32), (128, 64), (32, 128)}
+
+
 @pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3, ls.BF16, ls.BF16X6, ls.FP64])
 @pytest.mark.parametrize("shape", SHAPES)
 def test_planner_invariants(prec, shape):
@@ -213,7 +216,11 @@ def test_planner_invariants(prec, shape):
     for mp in (False, True):
         p = ls.query_plan(prec, M, N, K, num_cu=256, mp=mp)
         tm, tn = p["tile"]
-        assert (tm, tn) in {(32, 32), (64, 64), (128, 128), (128, 32), (128, 64), (32, 128)}
+        if (tm, tn) not in TILES:  # the FP32 short-side kernel: 16 long-side indices, K whole
+            assert prec == ls.FP32 and ((N <= 64 and M >= 1024) or (M <= 64 and N >= 1024)), (M, N, K, p)
+            assert (tm if N <= 64 else tn) == 16 and p["ksplit"] == 1 and p["kper"] == max(K, 1), (M, N, K, p)
+            continue
+        assert (tm, tn) in TILES
         if prec == ls.FP64:
             assert (tm, tn) != (128, 128)
         ks, kp = p["ksplit"], p["kper"]
@@ -226,7 +233,7 @@ def test_planner_invariants(prec, shape):
 
 def test_planner_fills_the_chip():
     # tall-skinny: few tiles, so the planner splits K
-    p = ls.query_plan(ls.FP32, 4096, 16, 1024, num_cu=256, mp=True)
+    p = ls.query_plan(ls.BF16X3, 4096, 16, 1024, num_cu=256, mp=True)
     tiles = -(-4096 // p["tile"][0]) * -(-16 // p["tile"][1])
     assert tiles * p["ksplit"] >= 64
     # big square: plenty of tiles, no split
@@ -265,9 +272,31 @@ def test_planner_keeps_waves_busy_on_skinny_grid(prec):
     shapes = list(TALL_SKINNY) + [(N, M, K) for (M, N, K) in TALL_SKINNY]
     for (M, N, K) in shapes:
         tile = ls.query_plan(prec, M, N, K, num_cu=256, mp=True)["tile"]
+        if tile not in WAVE_GRID:
+            continue  # the FP32 short-side kernel: its four waves split K (test below)
         if idle_fraction(tile, M, N, prec) > 0.5:
             others = [t for t in WAVE_GRID if not (prec == ls.FP64 and t == (128, 128))]
             assert all(idle_fraction(t, M, N, prec) > 0.5 for t in others), (M, N, K, tile)
+
+
+def test_short_side_plans():
+    """FP32 GEMMs with a short side (N <= 64 and M >= 1024, or M <= 64 and
+    N >= 1024) run on the short-side kernel: 16 long-side indices per
+    workgroup, the short side in 16/32/64-wide blocks, K whole; other modes
+    and squarer shapes keep the tile planner."""
+    for (M, N, K, want) in [(4096, 16, 1024, (16, 16)), (4096, 17, 1024, (16, 32)), (8192, 16, 1024, (16, 16)),
+                            (16, 4096, 1024, (16, 16)), (48, 4096, 1024, (64, 16)), (1024, 32, 1024, (16, 32)),
+                            (4096, 16, 4096, (16, 16)), (4096, 64, 512, (16, 64))]:
+        p = ls.query_plan(ls.FP32, M, N, K, num_cu=256, mp=True)
+        assert p["tile"] == want and p["ksplit"] == 1 and p["kper"] == K, (M, N, K, p)
+    # other modes, squarer shapes, small K, and long chains of K x short-side
+    # work per CU keep the tile planner
+    for (prec, M, N, K) in [(ls.BF16X3, 4096, 16, 1024), (ls.FP64, 4096, 16, 1024), (ls.FP32, 1000, 16, 1024),
+                            (ls.FP32, 4096, 65, 1024), (ls.FP32, 65, 4096, 1024), (ls.FP32, 4096, 16, 256),
+                            (ls.FP32, 1024, 64, 4096), (ls.FP32, 2048, 32, 2048), (ls.FP32, 65536, 64, 1024),
+                            (ls.FP32, 16, 16384, 1024)]:
+        p = ls.query_plan(prec, M, N, K, num_cu=256, mp=True)
+        assert p["tile"] in TILES or (prec == ls.FP64 and p["tile"] == (16, 16)), (prec, M, N, K, p)
 
 
 def test_planner_errors():

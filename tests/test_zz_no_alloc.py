@@ -37,8 +37,10 @@ def run_pair(dev, M, N, K, tb, prec, seed):
 
 @pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X6])
 def test_noalloc_matches_allocating_entry(dev, prec):
-    # both plans split K (4 and 2+ slices); the second call is smaller
-    for (M, N, K, tb, seed) in [(4096, 16, 1024, 0, 1), (1024, 32, 512, 1, 2)]:
+    # both plans split K (8 and 4 slices); the second call is smaller.  (M
+    # below 1024 keeps them off the FP32 short-side kernel, which does not
+    # split K and uses no scratch.)
+    for (M, N, K, tb, seed) in [(1000, 16, 1024, 0, 1), (512, 32, 512, 1, 2)]:
         plan = ls.query_plan(prec, M, N, K)
         assert plan["ksplit"] > 1, plan
         run_pair(dev, M, N, K, tb, prec, seed)
