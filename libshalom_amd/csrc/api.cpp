@@ -439,7 +439,14 @@ int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B,
     d.ldc = (int)ldc;
     const int nc = skinny_choice(prec, ta, tb, M, N, K, lda, ldb, info.num_cu);
     if (nc >= 0) {
-        SkinnyLaunch sk = skinny_table().fn[ta][tb][M <= 64][nc];
+        // eight waves (K in eighths) for a tall NN GEMM whose workgroups
+        // carry K x short-side work of 4096 in one round: 4096 x 64 x 1024
+        // 12.7 -> 10.3 us, 4096 x 16 x 4096 17.0 -> 14.7 us; the other
+        // cases measured the same or slower with them
+        const bool wide = M <= 64;
+        const long nwg = ((wide ? N : M) + 15) / 16;
+        const int nw8 = !wide && !ta && !tb && nwg <= info.num_cu && (K << nc) >= 4096;
+        SkinnyLaunch sk = skinny_table().fn[ta][tb][wide][nc][nw8];
         if (!sk) return LIBSHALOM_NOT_SUPPORTED;
         const SkinnyArgs a{(const float *)d.A, (const float *)d.B, (float *)C, (int)M, (int)N, (int)K,
                            (int)lda, (int)ldb, (int)ldc, (float)alpha, (float)beta};

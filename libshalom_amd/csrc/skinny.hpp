@@ -86,14 +86,14 @@ struct SkinnyOperand {
     }
 };
 
-template <int TA, int TB, int WIDE, int NC>
-__global__ __launch_bounds__(256) void k_gemm_skinny(SkinnyArgs a)
+template <int TA, int TB, int WIDE, int NC, int NW>
+__global__ __launch_bounds__(64 * NW) void k_gemm_skinny(SkinnyArgs a)
 {
     constexpr int D = skinny_depth<NC>();
     // X: k-contiguous when op(A) is A (tall) or op(B)^T is B stored N x K (wide)
     constexpr bool XKC = WIDE ? TB != 0 : TA == 0;
     constexpr bool YKC = WIDE ? TA == 0 : TB != 0;
-    __shared__ __attribute__((aligned(16))) float red[4][16][16 * NC];
+    __shared__ __attribute__((aligned(16))) float red[NW][16][16 * NC];
 
     const int lane = threadIdx.x & 63, wave = uniform((int)(threadIdx.x >> 6));
     const int r = lane & 15, q = lane >> 4, kq = 4 * q;
@@ -106,8 +106,8 @@ __global__ __launch_bounds__(256) void k_gemm_skinny(SkinnyArgs a)
     const float *yp = WIDE ? a.A : a.B;
     const int yld = WIDE ? a.lda : a.ldb;
 
-    // the wave's K range: quarters rounded up to whole groups of 16
-    const int kq4 = ((a.K + 63) / 64) * 16;
+    // the wave's K range: an NW-th, rounded up to whole groups of 16
+    const int kq4 = ((a.K + 16 * NW - 1) / (16 * NW)) * 16;
     const int kb = wave * kq4, ke = min(a.K, kb + kq4);
     const int G = ke > kb ? (ke - kb + 15) / 16 : 0;
 
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gemm_skinny(SkinnyArgs a)
         }
     }
 
-    // the four waves' partial sums meet in LDS: D[i][j] of wave w at red[w][i][j]
+    // the waves' partial sums meet in LDS: D[i][j] of wave w at red[w][i][j]
 #pragma unroll
     for (int c = 0; c < NC; ++c)
 #pragma unroll
@@ -162,11 +162,14 @@ __global__ __launch_bounds__(256) void k_gemm_skinny(SkinnyArgs a)
     // C[j][s0 + i], consecutive threads along i.
     constexpr int E = 16 * 16 * NC;
 #pragma unroll
-    for (int e0 = 0; e0 < E; e0 += 256) {
+    for (int e0 = 0; e0 < E; e0 += 64 * NW) {
         const int e = e0 + (int)threadIdx.x;
+        if (e >= E) break;
         const int i = WIDE ? e % 16 : e / (16 * NC), j = WIDE ? e / 16 : e % (16 * NC);
         if (s0 + i >= L || j >= S) continue;
-        const float sum = (red[0][i][j] + red[1][i][j]) + (red[2][i][j] + red[3][i][j]);
+        float sum = 0.f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) sum += red[w][i][j];
         float *o = WIDE ? a.C + (long long)j * a.ldc + (s0 + i) : a.C + (long long)(s0 + i) * a.ldc + j;
         *o = a.beta != 0.f ? a.alpha * sum + a.beta * *o : a.alpha * sum;
     }
@@ -174,17 +177,18 @@ __global__ __launch_bounds__(256) void k_gemm_skinny(SkinnyArgs a)
 
 typedef hipError_t (*SkinnyLaunch)(const SkinnyArgs &a, hipStream_t s);
 
-template <int TA, int TB, int WIDE, int NC>
+template <int TA, int TB, int WIDE, int NC, int NW>
 hipError_t launch_skinny(const SkinnyArgs &a, hipStream_t s)
 {
     const int L = WIDE ? a.N : a.M;
-    hipLaunchKernelGGL((k_gemm_skinny<TA, TB, WIDE, NC>), dim3((unsigned)((L + 15) / 16)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL((k_gemm_skinny<TA, TB, WIDE, NC, NW>), dim3((unsigned)((L + 15) / 16)), dim3(64 * NW), 0, s,
+                       a);
     return hipGetLastError();
 }
 
-// [ta][tb][wide][NC 1, 2, 4 -> 0, 1, 2]
+// [ta][tb][wide][NC 1, 2, 4 -> 0, 1, 2][NW 4, 8 -> 0, 1]
 struct SkinnyTable {
-    SkinnyLaunch fn[2][2][2][3];
+    SkinnyLaunch fn[2][2][2][3][2];
 };
 SkinnyTable &skinny_table();
 

@@ -19,8 +19,10 @@ U32 = 2.0 ** -24
 TRANS = [(0, 0), (0, 1), (1, 0), (1, 1)]
 # (M, N, K): tall and wide, long sides not multiples of 16, short sides of
 # 1 to 64 (one, two and four 16-column blocks), K ragged against 16 and 64
+# (the last two take the eight-wave kernel in NN: one round of workgroups
+# carrying 4096 of K x short side)
 SHAPES = [(1031, 5, 777), (2048, 17, 1024), (4093, 33, 520), (3000, 64, 512), (1024, 1, 2000),
-          (7, 1500, 1000), (40, 4096, 512), (64, 2047, 512), (1, 3000, 700)]
+          (7, 1500, 1000), (40, 4096, 512), (64, 2047, 512), (1, 3000, 700), (4090, 20, 2048), (4085, 7, 4096)]
 
 
 def is_short_side_plan(M, N, K):
