@@ -1419,6 +1419,9 @@ struct Tile {
         for (int a = 0; a < WM; ++a)
 #pragma unroll
             for (int b = 0; b < WN; ++b) {
+                // FP32 / FP64 skip the sub-tiles past the job's rows or
+                // columns (a prefix of the wave's; uniform branches)
+                if (Cfg::INTERLEAVE && (a >= wm || b >= wn)) continue;
                 const int col = Cfg::col_block(wc0, b) * 16 + r;
                 const int row = Cfg::row_block(wr0, a) * 16 + acc_row(q, 0);
                 const int off = (row * ldo + cm(col)) * ES_OUT | (col < N ? 0 : kOOB);
