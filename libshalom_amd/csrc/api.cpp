@@ -381,7 +381,7 @@ int noalloc_scratch(int dev, char **out)
     return LIBSHALOM_SUCCESS;
 }
 
-// The short-side kernel (skinny.hpp) for an FP32 GEMM whose C is tall (N <=
+// The short-side kernel (skinny.hpp) for a GEMM whose C is tall (N <=
 // 64) or wide (M <= 64) with a long side L of at least 1024, when it beats
 // the tile kernels' split-K plans (measured on MI355X, sgemm_mp shapes, A/B
 // against the tile plans): K >= 512 (below that both are launch-bound); the
@@ -391,7 +391,8 @@ int noalloc_scratch(int dev, char **out)
 // 35 us against 19; 2048 x 32 x 2048: 14.5 against 12.0); a wide C in one
 // round (16 x 16384 x 1024 ran slower).  Wins: 4096 x 16 x 1024 11.6 -> 6.3
 // us, 2048 x 16 x 1024 9.6 -> 6.1 us, 16 x 4096 x 1024 11.3 -> 7.0 us,
-// 4096 x 16 x 4096 21.8 -> 17.1 us.  And the operands' byte offsets (past
+// 4096 x 16 x 4096 21.8 -> 17.1 us; in BF16X3 4096 x 16 x 1024 11.7 -> 6.5 us,
+// 4096 x 32 x 1024 23.3 -> 16.0 us.  And the operands' byte offsets (past
 // their extents by at most a group of 16 indices) must stay in 31 bits.
 // Returns the kernel's column-block count index (0, 1, 2 for 16, 32, 64
 // short-side columns), or -1 for the tile kernels.  LIBSHALOM_SKINNY=0
@@ -402,7 +403,7 @@ int skinny_choice(int prec, int ta, int tb, long M, long N, long K, long lda, lo
         const char *e = std::getenv("LIBSHALOM_SKINNY");
         return e && e[0] == '0';
     }();
-    if (off || prec != P_FP32) return -1;
+    if (off || prec == P_FP64) return -1;
     const bool wide = M <= 64 && N >= 1024, tall = N <= 64 && M >= 1024;
     if (!wide && !tall) return -1;
     const long L = wide ? N : M, S = wide ? M : N;
@@ -417,6 +418,10 @@ int skinny_choice(int prec, int ta, int tb, long M, long N, long K, long lda, lo
     const int nc = S <= 16 ? 0 : S <= 32 ? 1 : 2;
     const long nwg = (L + 15) / 16, rounds = (nwg + num_cu - 1) / num_cu, chain = K << nc;
     if (K < 512 || rounds * chain > 4096 || (nwg < num_cu && chain > 2048) || (wide && rounds > 1)) return -1;
+    // the split modes' tile plans keep up on short sides above 32 (their
+    // MFMA work per byte is a third of FP32's): 4096 x 64 x 1024 NT and
+    // 64 x 4096 x 1024 NT ran 10-15% slower on the short-side kernel
+    if (prec != P_FP32 && nc == 2) return -1;
     return nc;
 }
 
@@ -445,8 +450,8 @@ int run_general(int prec, int ta, int tb, void *C, const void *A, const void *B,
         // cases measured the same or slower with them
         const bool wide = M <= 64;
         const long nwg = ((wide ? N : M) + 15) / 16;
-        const int nw8 = !wide && !ta && !tb && nwg <= info.num_cu && (K << nc) >= 4096;
-        SkinnyLaunch sk = skinny_table().fn[ta][tb][wide][nc][nw8];
+        const int nw8 = prec == P_FP32 && !wide && !ta && !tb && nwg <= info.num_cu && (K << nc) >= 4096;
+        SkinnyLaunch sk = skinny_table().fn[prec][ta][tb][wide][nc][nw8];
         if (!sk) return LIBSHALOM_NOT_SUPPORTED;
         const SkinnyArgs a{(const float *)d.A, (const float *)d.B, (float *)C, (int)M, (int)N, (int)K,
                            (int)lda, (int)ldb, (int)ldc, (float)alpha, (float)beta};

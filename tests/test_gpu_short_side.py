@@ -1,11 +1,12 @@
-"""GPU parity of the FP32 short-side kernel (a tall C with N <= 64 or a wide
-one with M <= 64, K >= 512): every transpose combination, ragged long and
-short sides, K not a multiple of the 16-k groups or of the four waves'
-quarters, alpha / beta with padded leading dimensions.  Each case first
-checks (query_plan) that the planner routes it to the short-side kernel;
-every element of C is checked against a float64 product on the device and a
-sample of rows against the oracle, C prefilled with NaN (padding must stay
-untouched)."""
+"""GPU parity of the short-side kernel (a tall C with N <= 64 or a wide one
+with M <= 64, K >= 512), FP32 and the split-BF16 modes: every transpose
+combination, ragged long and short sides, K not a multiple of the 16- (32-)
+k groups or of the waves' shares, alpha / beta with padded leading
+dimensions.  Each case first checks (query_plan) that the planner routes it
+to the short-side kernel; every element of C is checked against a float64
+product on the device (FP32 and BF16X6 to 2 K u |A||B|, BF16X3 and BF16
+with their per-product terms) and, in FP32, a sample of rows against the
+oracle, C prefilled with NaN (padding must stay untouched)."""
 import numpy as np
 import pytest
 import torch
@@ -17,6 +18,8 @@ pytestmark = pytest.mark.gpu
 
 U32 = 2.0 ** -24
 TRANS = [(0, 0), (0, 1), (1, 0), (1, 1)]
+# extra relative error per product |a||b| of the one- and two-piece splits
+EXTRA = {ls.FP32: 0.0, ls.BF16X6: 0.0, ls.BF16X3: 3 * 2.0 ** -16, ls.BF16: 2 * 2.0 ** -8}
 # (M, N, K): tall and wide, long sides not multiples of 16, short sides of
 # 1 to 64 (one, two and four 16-column blocks), K ragged against 16 and 64
 # (the last two take the eight-wave kernel in NN: one round of workgroups
@@ -25,12 +28,12 @@ SHAPES = [(1031, 5, 777), (2048, 17, 1024), (4093, 33, 520), (3000, 64, 512), (1
           (7, 1500, 1000), (40, 4096, 512), (64, 2047, 512), (1, 3000, 700), (4090, 20, 2048), (4085, 7, 4096)]
 
 
-def is_short_side_plan(M, N, K):
-    p = ls.query_plan(ls.FP32, M, N, K, num_cu=torch.cuda.get_device_properties(0).multi_processor_count)
+def is_short_side_plan(M, N, K, prec=ls.FP32):
+    p = ls.query_plan(prec, M, N, K, num_cu=torch.cuda.get_device_properties(0).multi_processor_count)
     return p["ksplit"] == 1 and 16 in p["tile"] and p["tile"] not in {(32, 32), (64, 64), (128, 128)}
 
 
-def run_general(dev, ta, tb, M, N, K, alpha, beta, pad, seed):
+def run_general(dev, ta, tb, M, N, K, alpha, beta, pad, seed, prec=ls.FP32):
     r = oracle.Drand48(seed)
     lda = (M if ta else K) + pad
     ldb = (K if tb else N) + pad
@@ -43,7 +46,7 @@ def run_general(dev, ta, tb, M, N, K, alpha, beta, pad, seed):
     dC = torch.from_numpy(C0.copy()).to(dev)
     if beta == 0.0:
         dC[:, :N] = float("nan")  # not read when beta is 0
-    ls.sgemm_general(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, ls.FP32)
+    ls.sgemm_general(ta, tb, M, N, K, alpha, dA, lda, dB, ldb, beta, dC, ldc, prec)
     torch.cuda.synchronize()
     C = dC.cpu().numpy()
     assert np.array_equal(C[:, N:], C0[:, N:]), "padding written"
@@ -56,10 +59,10 @@ def run_general(dev, ta, tb, M, N, K, alpha, beta, pad, seed):
     ref = alpha * (tA @ tB) + beta * tC0
     den = abs(alpha) * (tA.abs() @ tB.abs()) + abs(beta) * tC0.abs()
     err = (torch.from_numpy(C[:, :N].astype(np.float64)).to(dev) - ref).abs()
-    bad = int((~(err <= 2 * (K + 2) * U32 * den)).sum())
-    assert bad == 0, (ta, tb, M, N, K, alpha, beta, bad)
-    # a sample of rows against the oracle (alpha = 1, beta = 0 cases)
-    if alpha == 1.0 and beta == 0.0:
+    bad = int((~(err <= (2 * (K + 2) * U32 + EXTRA[prec]) * den)).sum())
+    assert bad == 0, (prec, ta, tb, M, N, K, alpha, beta, bad)
+    # a sample of rows against the oracle (FP32, alpha = 1, beta = 0 cases)
+    if prec == ls.FP32 and alpha == 1.0 and beta == 0.0:
         rows = np.unique(np.linspace(0, M - 1, num=min(M, 24)).astype(int))
         Asub = np.ascontiguousarray(opA[rows].astype(np.float32))
         Bm = np.ascontiguousarray(opB.astype(np.float32))
@@ -82,6 +85,16 @@ def test_short_side_alpha_beta_padded(dev, ta, tb):
         assert is_short_side_plan(M, N, K), (M, N, K)
         for alpha, beta in [(0.5, 0.0), (-2.0, 1.5)]:
             run_general(dev, ta, tb, M, N, K, alpha, beta, 3, 200 + i)
+
+
+@pytest.mark.parametrize("prec", [ls.BF16X3, ls.BF16, ls.BF16X6])
+@pytest.mark.parametrize("ta,tb", TRANS)
+def test_short_side_split_modes(dev, prec, ta, tb):
+    for i, (M, N, K) in enumerate([(1031, 5, 777), (4093, 31, 520), (7, 1500, 1000), (32, 2047, 512),
+                                   (2048, 17, 1024)]):
+        assert is_short_side_plan(M, N, K, prec), (prec, M, N, K)
+        run_general(dev, ta, tb, M, N, K, 1.0, 0.0, 0, 300 + i, prec)
+    run_general(dev, ta, tb, 1500, 24, 640, -2.0, 1.5, 3, 310, prec)
 
 
 def test_short_side_reference_api_matches_general(dev):

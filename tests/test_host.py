@@ -216,8 +216,8 @@ def test_planner_invariants(prec, shape):
     for mp in (False, True):
         p = ls.query_plan(prec, M, N, K, num_cu=256, mp=mp)
         tm, tn = p["tile"]
-        if (tm, tn) not in TILES:  # the FP32 short-side kernel: 16 long-side indices, K whole
-            assert prec == ls.FP32 and ((N <= 64 and M >= 1024) or (M <= 64 and N >= 1024)), (M, N, K, p)
+        if (tm, tn) not in TILES:  # the short-side kernel: 16 long-side indices, K whole
+            assert prec != ls.FP64 and ((N <= 64 and M >= 1024) or (M <= 64 and N >= 1024)), (M, N, K, p)
             assert (tm if N <= 64 else tn) == 16 and p["ksplit"] == 1 and p["kper"] == max(K, 1), (M, N, K, p)
             continue
         assert (tm, tn) in TILES
@@ -280,18 +280,22 @@ def test_planner_keeps_waves_busy_on_skinny_grid(prec):
 
 
 def test_short_side_plans():
-    """FP32 GEMMs with a short side (N <= 64 and M >= 1024, or M <= 64 and
-    N >= 1024) run on the short-side kernel: 16 long-side indices per
-    workgroup, the short side in 16/32/64-wide blocks, K whole; other modes
-    and squarer shapes keep the tile planner."""
-    for (M, N, K, want) in [(4096, 16, 1024, (16, 16)), (4096, 17, 1024, (16, 32)), (8192, 16, 1024, (16, 16)),
-                            (16, 4096, 1024, (16, 16)), (48, 4096, 1024, (64, 16)), (1024, 32, 1024, (16, 32)),
-                            (4096, 16, 4096, (16, 16)), (4096, 64, 512, (16, 64))]:
-        p = ls.query_plan(ls.FP32, M, N, K, num_cu=256, mp=True)
-        assert p["tile"] == want and p["ksplit"] == 1 and p["kper"] == K, (M, N, K, p)
-    # other modes, squarer shapes, small K, and long chains of K x short-side
-    # work per CU keep the tile planner
-    for (prec, M, N, K) in [(ls.BF16X3, 4096, 16, 1024), (ls.FP64, 4096, 16, 1024), (ls.FP32, 1000, 16, 1024),
+    """FP32 and split-mode GEMMs with a short side (N <= 64 and M >= 1024, or
+    M <= 64 and N >= 1024) run on the short-side kernel: 16 long-side indices
+    per workgroup, the short side in 16/32/64-wide blocks, K whole; FP64 and
+    squarer shapes keep the tile planner."""
+    for prec in (ls.FP32, ls.BF16X3, ls.BF16, ls.BF16X6):
+        for (M, N, K, want) in [(4096, 16, 1024, (16, 16)), (4096, 17, 1024, (16, 32)), (8192, 16, 1024, (16, 16)),
+                                (16, 4096, 1024, (16, 16)), (48, 4096, 1024, (64, 16)), (1024, 32, 1024, (16, 32)),
+                                (4096, 16, 4096, (16, 16)), (4096, 64, 512, (16, 64))]:
+            if prec != ls.FP32 and 64 in want:
+                continue  # the split modes keep their tile plans on short sides above 32
+            p = ls.query_plan(prec, M, N, K, num_cu=256, mp=True)
+            assert p["tile"] == want and p["ksplit"] == 1 and p["kper"] == K, (prec, M, N, K, p)
+    # FP64, squarer shapes, small K, long chains of K x short-side work per
+    # CU, and split-mode short sides above 32 keep the tile planner
+    for (prec, M, N, K) in [(ls.FP64, 4096, 16, 1024), (ls.FP32, 1000, 16, 1024), (ls.BF16X3, 4096, 64, 1024),
+                            (ls.BF16X6, 48, 4096, 1024),
                             (ls.FP32, 4096, 65, 1024), (ls.FP32, 65, 4096, 1024), (ls.FP32, 4096, 16, 256),
                             (ls.FP32, 1024, 64, 4096), (ls.FP32, 2048, 32, 2048), (ls.FP32, 65536, 64, 1024),
                             (ls.FP32, 16, 16384, 1024)]:
