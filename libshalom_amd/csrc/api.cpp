@@ -580,10 +580,11 @@ void bwd_slice(BwdProblem &P, int T, long len)
 // at least big_quarters / 4 of the compute units a unit of work (reduction
 // slices fill the rest), else 64 x 64.  (Measured on the conv layers with
 // three quarters: the 128x784 layer, 224 big tiles for 256 CUs, runs
-// 1.1-1.35x faster on them; the 256x196 layer forward, 128 big tiles, runs
-// slower, FP32 by 1.5x -- but its implicit backward, whose dW problem adds
-// tiles, 1.2x faster in BF16X3 (and 7% slower in FP32), so that launch plans
-// with one half for the split-BF16 modes.)  A reduction is cut into
+// 1.1-1.35x faster on them.  The 256x196 layer, 128 big tiles, runs its
+// FP32 forward 3% slower on them (with two slices per tile) and its
+// implicit backward 7% slower, but in BF16X3 its stored forward 1.16x, its
+// implicit forward 1.27x and its implicit backward 1.2x faster, so the
+// split-BF16 modes plan with one half.)  A reduction is cut into
 // slices of about W / (2 * CUs) for 64 x 64 tiles, W / CUs for 128 x 128
 // tiles (W = all tiles' reduction work) so that no unit is much longer than
 // a compute unit's share: a 128 x 128 workgroup fills its CU alone, and one
@@ -648,7 +649,12 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
     // estimated makespan: rounds of units over the resident workgroups (one
     // per CU for 128 x 128 tiles, occ64 for 64 x 64) times the longest
     // unit's reduction steps; ties go to the longer slices (fewer partial
-    // tiles).  (The 7x7 conv layer's backward, 300 units of 128 x 128 for
+    // tiles) unless those leave more than a quarter of the resident
+    // workgroups without a unit where the shorter ones did not (the 14x14
+    // conv layer's implicit backward, BF16X3: 170 units of 128 x 128 in one
+    // round against 340 in two, 0.175 -> 0.155 ms; the 28x28 layer's
+    // forward keeps its 224 unsliced units against 448 halves, 0.038 against
+    // 0.058 ms).  (The 7x7 conv layer's backward, 300 units of 128 x 128 for
     // 256 CUs at the base length, runs in one round of 248 at twice it:
     // 0.21 -> 0.15 ms.  Single-problem launches -- the strided and the
     // implicit-convolution forward -- plan with four resident workgroups:
@@ -659,6 +665,7 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
     const long resident = (long)num_cu * (T == 128 ? 1 : occ64);
     long best = len0;
     double best_cost = 0;
+    bool best_fills = false;
     const int mlo = occ64 > 2 ? 1 : 4;  // quarters of len0: from a quarter with more resident units
     for (int m = mlo; m <= 16; m *= 2) {
         const long lenm = std::max(256L, ceil_div(len0 * m / 4, 32) * 32);
@@ -669,9 +676,11 @@ int plan_problems(BwdProblem **probs, int n, bool allow_big, int dev, int num_cu
             if (P.units > 0) longest = std::max(longest, P.spi > 0 ? (double)P.slen : (double)P.gimg * P.red);
         }
         const double cost = (double)ceil_div(units, resident) * longest;
-        if (m == mlo || cost <= best_cost) {
+        const bool fills = 4 * units >= 3 * resident;
+        if (m == mlo || cost < best_cost || (cost == best_cost && (fills || !best_fills))) {
             best = lenm;
             best_cost = cost;
+            best_fills = fills;
         }
     }
     slice_all(best);
@@ -733,7 +742,7 @@ int strided_run(int prec, int ta, int tb, float *C, const float *A, const float 
     bwd_side(P, A, sA, (int)(ta ? M : K), B, sB, (int)(tb ? K : N), C, sC, (int)N, M, N, K, nimg);
     BwdProblem *probs[1] = {&P};
     // one problem: its 64 x 64 workgroups are light enough for four per CU
-    const int T = plan_problems(probs, 1, std::min(M, N) >= 128, dev, info.num_cu, s, 3, 4);
+    const int T = plan_problems(probs, 1, std::min(M, N) >= 128, dev, info.num_cu, s, prec == P_FP32 ? 3 : 2, 4);
     if (T <= 0) return -T;
     StridedLaunch fn = bwd_table().strided[prec][ta][tb][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
@@ -895,7 +904,8 @@ int conv_fwd_run(int prec, float *Y, const float *Wt, const float *X, long batch
     bwd_side(Pb, Wt, 0, (int)K, X, C * H * W, 0, Y, Cout * PQ, (int)PQ, Cout, PQ, K, (int)batch);
     Pb.kalign = (int)(32 / std::gcd(32L, RS) * RS);
     BwdProblem *probs[1] = {&Pb};
-    const int T = plan_problems(probs, 1, std::min(Cout, (long)Pb.Cc) >= 128, dev, info.num_cu, s, 3, 4);
+    const int T = plan_problems(probs, 1, std::min(Cout, (long)Pb.Cc) >= 128, dev, info.num_cu, s,
+                                prec == P_FP32 ? 3 : 2, 4);
     if (T <= 0) return -T;
     ConvLaunch fn = bwd_table().conv_fwd[prec][T == 128];
     if (!fn) return LIBSHALOM_NOT_SUPPORTED;
