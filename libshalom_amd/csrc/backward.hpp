@@ -294,19 +294,44 @@ struct BwdLds {
     static constexpr int BYTES = CX::LDS_BYTES > CY::LDS_BYTES ? CX::LDS_BYTES : CY::LDS_BYTES;
 };
 
-// Workgroups [0, X.grid rounded up to a multiple of 8) are dA units, the rest
-// dB units.
+// Which problem workgroup u of a two-problem launch works on, and its unit
+// there.  The problem with the longer units comes first (yfirst: Y), so that
+// those start in the first round rather than behind the other's; the second
+// starts on a multiple of 8, so that a unit's index keeps its XCD.
+struct TwoUnits {
+    bool x;
+    int u;
+};
+__device__ __forceinline__ TwoUnits two_units(const BwdProblem &X, const BwdProblem &Y, int yfirst, int u)
+{
+    const int xg = (X.grid + 7) & ~7, yg = (Y.grid + 7) & ~7;
+    if (yfirst) return u >= yg ? TwoUnits{true, u - yg} : TwoUnits{false, u};
+    return u < xg ? TwoUnits{true, u} : TwoUnits{false, u - xg};
+}
+
+// Reduction steps of a problem's longest unit (0: none); a two-problem
+// launch puts the problem with the longer ones first.
+inline long longest_unit(const BwdProblem &P)
+{
+    return P.units <= 0 ? 0 : P.spi > 0 ? (long)P.slen : (long)P.gimg * P.red;
+}
+inline int launch_grid(const BwdProblem &X, const BwdProblem &Y, int yfirst)
+{
+    return yfirst ? ((Y.grid + 7) & ~7) + X.grid : ((X.grid + 7) & ~7) + Y.grid;
+}
+
+// dA units and dB units in one launch (two_units).
 template <class CX, class CY>
 __global__ __attribute__((amdgpu_flat_work_group_size(CX::NT, CX::NT))) void
-k_gemm_backward(BwdProblem X, BwdProblem Y)
+k_gemm_backward(BwdProblem X, BwdProblem Y, int yfirst)
 {
     __shared__ __attribute__((aligned(16))) char lds[BwdLds<CX, CY>::BYTES];
     __shared__ int flag;
-    const int u = blockIdx.x, xg = (X.grid + 7) & ~7;  // Y starts on a multiple of 8
-    if (u < xg)
-        bwd_unit<CX>(X, 0, u, lds, &flag);
+    const TwoUnits w = two_units(X, Y, yfirst, blockIdx.x);
+    if (w.x)
+        bwd_unit<CX>(X, 0, w.u, lds, &flag);
     else
-        bwd_unit<CY>(Y, offsetof(ArgsXY, Y), u - xg, lds, &flag);
+        bwd_unit<CY>(Y, offsetof(ArgsXY, Y), w.u, lds, &flag);
 }
 
 // One problem on its own: a strided batch of GEMMs C_i = op(A_i) op(B_i)
@@ -343,15 +368,15 @@ k_conv_fwd(BwdProblem P, ConvGeom g)
 // the sum over images and pixels cut into slices as for a shared operand.
 template <class CX, class CY, int IMY>
 __global__ __attribute__((amdgpu_flat_work_group_size(CX::NT, CX::NT))) void
-k_conv_backward(BwdProblem X, BwdProblem Y, ConvGeom gx, ConvGeom gy)
+k_conv_backward(BwdProblem X, BwdProblem Y, ConvGeom gx, ConvGeom gy, int yfirst)
 {
     __shared__ __attribute__((aligned(16))) char lds[BwdLds<CX, CY>::BYTES];
     __shared__ int flag;
-    const int u = blockIdx.x, xg = (X.grid + 7) & ~7;  // Y starts on a multiple of 8
-    if (u < xg)
-        bwd_unit<CX, kIm2col>(X, 0, u, lds, &flag, gx, offsetof(ArgsXYGG, gx));
+    const TwoUnits w = two_units(X, Y, yfirst, blockIdx.x);
+    if (w.x)
+        bwd_unit<CX, kIm2col>(X, 0, w.u, lds, &flag, gx, offsetof(ArgsXYGG, gx));
     else
-        bwd_unit<CY, IMY>(Y, offsetof(ArgsXY, Y), u - xg, lds, &flag, gy, offsetof(ArgsXYGG, gy));
+        bwd_unit<CY, IMY>(Y, offsetof(ArgsXY, Y), w.u, lds, &flag, gy, offsetof(ArgsXYGG, gy));
 }
 
 // ---------------------------------------------------------------------------
@@ -372,9 +397,9 @@ hipError_t launch_conv_backward(const BwdProblem &X, const BwdProblem &Y, const 
     typedef TileCfg<PREC, 0, 1, BT, BT> CY;  // dW: dY x im2col(X)^T
     static_assert(CX::NT == CY::NT, "one workgroup size for both problems");
     if (X.units + Y.units <= 0) return hipSuccess;
-    const int grid = ((X.grid + 7) & ~7) + Y.grid;
-    hipLaunchKernelGGL((k_conv_backward<CX, CY, RUNS ? kIm2colTR : kIm2colT>), dim3(grid), dim3(CX::NT), 0, s, X, Y,
-                       gx, gy);
+    const int yfirst = longest_unit(Y) > longest_unit(X);
+    hipLaunchKernelGGL((k_conv_backward<CX, CY, RUNS ? kIm2colTR : kIm2colT>), dim3(launch_grid(X, Y, yfirst)),
+                       dim3(CX::NT), 0, s, X, Y, gx, gy, yfirst);
     return hipGetLastError();
 }
 
@@ -407,9 +432,9 @@ hipError_t launch_backward(const BwdProblem &X, const BwdProblem &Y, hipStream_t
     typedef TileCfg<PREC, kBwdTA[TA][TB][0], kBwdTB[TA][TB][0], BT, BT> CX;
     typedef TileCfg<PREC, kBwdTA[TA][TB][1], kBwdTB[TA][TB][1], BT, BT> CY;
     if (X.units + Y.units <= 0) return hipSuccess;
-    const int grid = ((X.grid + 7) & ~7) + Y.grid;
     static_assert(CX::NT == CY::NT, "one workgroup size for both problems");
-    hipLaunchKernelGGL((k_gemm_backward<CX, CY>), dim3(grid), dim3(CX::NT), 0, s, X, Y);
+    const int yfirst = longest_unit(Y) > longest_unit(X);
+    hipLaunchKernelGGL((k_gemm_backward<CX, CY>), dim3(launch_grid(X, Y, yfirst)), dim3(CX::NT), 0, s, X, Y, yfirst);
     return hipGetLastError();
 }
 
