@@ -130,7 +130,15 @@ __global__ __launch_bounds__(64 * NW) void k_gemm_skinny(SkinnyArgs a)
     const int r = lane & 15, q = lane >> 4, kq = V * q;
     const int L = WIDE ? a.N : a.M;  // long side
     const int S = WIDE ? a.M : a.N;  // short side
-    const int s0 = blockIdx.x * 16;
+    // an index-contiguous X is read in 64-byte runs: the workgroups of two
+    // neighbouring 16-index blocks share an XCD (workgroup w runs on XCD
+    // w % 8), so that the two halves of a 128-byte line meet in one L2
+    int blk = blockIdx.x;
+    if constexpr (!XKC) {
+        const int grp = blk >> 4, o = blk & 15;
+        if ((grp + 1) * 16 <= (int)gridDim.x) blk = grp * 16 + 2 * (o & 7) + (o >> 3);
+    }
+    const int s0 = blk * 16;
     // X: op(A) (tall) or op(B)^T (wide), index = long-side index
     const float *xp = WIDE ? a.B : a.A;
     const int xld = WIDE ? a.ldb : a.lda;
