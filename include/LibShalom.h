@@ -77,7 +77,9 @@ void LibShalom_sgemm(int transa, int transb, float *C, float *A, float *B, long 
 
 /* Replaces LibShalom_sgemm_mp (NN_LIB/LibShalom.h:27, SMM_thread.c:75-114):
  * the irregular-shape path (tall-skinny / short-wide); the OpenMP Tm x Tn
- * thread grid becomes a workgroup schedule with split-K. */
+ * thread grid becomes a workgroup schedule with split-K, or, for a short
+ * side of at most 64 and a long K, workgroups of 16 long-side indices that
+ * split K between their waves (one launch, no scratch). */
 void LibShalom_sgemm_mp(int transa, int transb, float *C, float *A, float *B, long M, long N, long K);
 
 /* Replaces LibShalom_dgemm (NN_LIB/LibShalom.h:39, NN_SMM.c:30-38). */
@@ -268,11 +270,12 @@ int LibShalom_query_tuned_tile(int precision, int transa, int transb, long M, lo
  * use for one GEMM on a device with num_cu compute units (mp != 0: the
  * LibShalom_sgemm_mp path), and the tile a batched plan gives one GEMM
  * (FP64 plans run the GEMMs of the 32 x 32 bucket with M, N <= 16 on
- * one-wave 16 x 16 tiles; both tile queries report that).  An FP32 GEMM
- * with a short side (N <= 64 and M >= 1024, or M <= 64 and N >= 1024) and a
- * moderate K may run on the short-side kernel instead, 16 long-side indices
- * per workgroup over all of K: reported as a 16 x 16/32/64 (or 16/32/64 x
- * 16) tile and no split.
+ * one-wave 16 x 16 tiles; both tile queries report that).  A GEMM with a
+ * short side (N <= 64 and M >= 1024, or M <= 64 and N >= 1024; in the
+ * split-BF16 modes a short side of at most 32) and a moderate K may run on
+ * the short-side kernel instead, 16 long-side indices per workgroup over
+ * all of K: reported as a 16 x 16/32/64 (or 16/32/64 x 16) tile and no
+ * split.
  * Replaces the reference's thread-grid planner Dete_grad_N_threads_nums /
  * Dete_grad_M_threads_nums (SMM_thread.c:14-72). */
 int LibShalom_query_plan(int precision, int mp, long M, long N, long K, int num_cu, int *tile_m,
