@@ -190,6 +190,9 @@ def main():
     log(f"dominant kernel ({vmode}, 128x128 bucket, {len(big)} GEMMs): {kms:.3f} ms/launch, {kgbs:.0f} GB/s, "
         f"{kflops / (kms * 1e-3) / 1e12:.1f} TFLOP/s")
     traffic = recorded_traffic(vmode)
+    copy_gbs = copy_rate(dev, stream)
+    log(f"device copy (1 GiB, torch copy_): {copy_gbs:.0f} GB/s read + write; the dominant kernel moves "
+        f"{kgbs / copy_gbs:.3f} of that")
 
     cpu = None
     if not args.no_cpu:
@@ -230,7 +233,8 @@ def main():
                      "kernel": f"k_gemm_persistent<{vmode}, NN, 128x128> (the {len(big)} GEMMs the plan puts in "
                                f"128x128 tiles)",
                      "kernel_ms": round(kms, 4), "kernel_algorithmic_bytes": kbytes,
-                     "kernel_tflops": round(kflops / (kms * 1e-3) / 1e12, 2)},
+                     "kernel_tflops": round(kflops / (kms * 1e-3) / 1e12, 2),
+                     "copy_gbs": round(copy_gbs, 1), "frac_of_copy": round(kgbs / copy_gbs, 3)},
         "cpu_baseline": cpu,
         "configs": configs,
     }
@@ -407,6 +411,20 @@ def time_ms(fn, steps, warmup, stream):
     torch.cuda.synchronize()
     _last_busy[0] = time.perf_counter()
     return e0.elapsed_time(e1) / steps
+
+
+def copy_rate(dev, stream):
+    """GB/s (bytes read + bytes written) of a plain 1 GiB device-to-device
+    copy on this GPU: what a kernel with no arithmetic reaches, beside the
+    8 TB/s peak the roofline fraction is taken against."""
+    import torch
+    n = 256 << 20
+    src = torch.ones(n, device=dev)
+    dst = torch.empty_like(src)
+    ms = time_ms(lambda: dst.copy_(src), 10, 2, stream)
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * 4 * n / (ms * 1e-3) / 1e9
 
 
 def roof_tflops(mode, flops, nbytes):
