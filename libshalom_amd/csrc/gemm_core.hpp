@@ -299,7 +299,9 @@ struct PanelInfo {
 // ---------------------------------------------------------------------------
 // Configuration of one tile kernel.
 // ---------------------------------------------------------------------------
-template <int PREC_, int TA_, int TB_, int BM_, int BN_>
+// PERSIST_: the configuration of the persistent batched kernel (its LDS
+// buffering may differ, below).
+template <int PREC_, int TA_, int TB_, int BM_, int BN_, bool PERSIST_ = false>
 struct TileCfg {
     static constexpr int PREC = PREC_, TA = TA_, TB = TB_, BM = BM_, BN = BN_;
     typedef typename PrecInfo<PREC>::T T;
@@ -315,7 +317,15 @@ struct TileCfg {
     static constexpr int A_BYTES = PA::BYTES;
     static constexpr int B_BYTES = PB::BYTES;
     static constexpr int BUF_BYTES = A_BYTES + B_BYTES;
-    static constexpr int NBUF = (2 * BUF_BYTES <= 96 * 1024) ? 2 : 1;
+    // Two LDS buffers (the next chunk written while the current one is
+    // multiplied) where they fit twice per CU.  BF16X6's persistent 128 x 128
+    // kernel, whose three bf16 planes take 48 KB a buffer, keeps one so that
+    // two workgroups share a CU: on the 50k U{1..128} batch its launch ran
+    // 0.738 ms with two buffers and one workgroup per CU, 0.583 ms with one
+    // buffer and two.  (The single-GEMM and slice kernels hold more than 128
+    // registers, so one workgroup per CU, and keep both buffers.)
+    static constexpr int NBUF = (PERSIST_ && PREC_ == P_BF16X6 && BM_ == 128 && BN_ == 128) ? 1
+                                : (2 * BUF_BYTES <= 96 * 1024) ? 2 : 1;
     static constexpr int LDS_BYTES = NBUF * BUF_BYTES;
     // Workgroup size: eight waves for the 128 x 128 tiles, four otherwise.
     // The waves form a GM x GN grid over the tile, each owning WM x WN
@@ -368,11 +378,11 @@ struct TileCfg {
     }
     static_assert(NW != 8 || (GM == 4 && GN == 2), "the eight-wave placement assumes a 4 x 2 grid");
     // Waves per SIMD the persistent kernel is compiled for: four for the
-    // eight-wave tiles of FP32 / BF16 / BF16X3 (two workgroups per CU, at
-    // most 128 registers), two for the other 128 x 128 tiles.  (The
+    // eight-wave tiles of FP32 / BF16 / BF16X3 / BF16X6 (two workgroups per
+    // CU, at most 128 registers), two for the other 128 x 128 tiles.  (The
     // single-GEMM tile kernel, with its alpha / beta epilogue, would spill
     // under the four-wave limit and keeps the default.)
-    static constexpr int WAVES_PER_EU = (NT == 512 && PREC != P_BF16X6 && PREC != P_FP64) ? 4
+    static constexpr int WAVES_PER_EU = (NT == 512 && PREC != P_FP64) ? 4
                                         : (BM * BN >= 128 * 128 ? 2 : 1);
     typedef typename std::conditional<sizeof(T) == 8, f64x4, f32x4>::type Acc;
     static_assert(BM % 16 == 0 && BN % 16 == 0, "tile must be a multiple of 16");

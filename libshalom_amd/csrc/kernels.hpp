@@ -259,11 +259,12 @@ template <int PREC, int TA, int TB, int BM, int BN>
 void register_one(LaunchTable &t, int tile)
 {
     typedef TileCfg<PREC, TA, TB, BM, BN> Cfg;
+    typedef TileCfg<PREC, TA, TB, BM, BN, true> PCfg;
     t.tiles[PREC][TA][TB][tile] = &launch_tiles<Cfg>;
     // the batched buckets use the square tiles
     if (BM == BN) {
-        t.persist[PREC][TA][TB][tile] = &launch_persistent<Cfg>;
-        t.persist_occ[PREC][TA][TB][tile] = &persistent_occupancy<Cfg>;
+        t.persist[PREC][TA][TB][tile] = &launch_persistent<PCfg>;
+        t.persist_occ[PREC][TA][TB][tile] = &persistent_occupancy<PCfg>;
     }
 }
 
@@ -277,7 +278,7 @@ void register_trans(LaunchTable &t)
     register_one<PREC, TA, TB, 128, 64>(t, T128x64);
     register_one<PREC, TA, TB, 32, 128>(t, T32x128);
     if constexpr (PREC == P_FP64) {
-        typedef TileCfg<PREC, TA, TB, 16, 16> C16;
+        typedef TileCfg<PREC, TA, TB, 16, 16, true> C16;
         t.persist16[TA][TB] = &launch_persistent<C16>;
         t.persist16_occ[TA][TB] = &persistent_occupancy<C16>;
     }
