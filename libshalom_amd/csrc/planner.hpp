@@ -61,9 +61,10 @@ inline long cdiv(long a, long b) { return (a + b - 1) / b; }
 
 // MFMA cycles one K chunk of a tile costs on its busiest SIMD (waves w and
 // w + 4 of an eight-wave tile share SIMD w % 4), for a GEMM covering mt x nt
-// 16 x 16 blocks of it.  FP32 / FP64 waves multiply the sub-tiles that reach
-// into the GEMM; the split modes every sub-tile of a wave holding part of
-// it.  The wave placement is the kernels' own (TileCfg::wave_g, rows_in, ...).
+// 16 x 16 blocks of it.  Waves multiply the sub-tiles that reach into the
+// GEMM (with contiguous placement, every sub-tile of a wave holding part of
+// it).  The wave placement is the kernels' own (TileCfg::wave_g, rows_in,
+// ...).
 template <class C>
 inline double busiest_simd(long M, long N)
 {
@@ -71,7 +72,7 @@ inline double busiest_simd(long M, long N)
     double simd[4] = {0, 0, 0, 0};
     for (int w = 0; w < C::NW; ++w) {
         const long rows = C::rows_in(C::wave_g(w), mt), cols = C::cols_in(C::wave_h(w), nt);
-        simd[w % 4] += rows * cols == 0 ? 0.0 : C::SPLIT ? (double)C::WM * C::WN : (double)rows * cols;
+        simd[w % 4] += rows * cols == 0 ? 0.0 : C::INTERLEAVE ? (double)rows * cols : (double)C::WM * C::WN;
     }
     return *std::max_element(simd, simd + 4);
 }
