@@ -299,8 +299,8 @@ struct PanelInfo {
 // ---------------------------------------------------------------------------
 // Configuration of one tile kernel.
 // ---------------------------------------------------------------------------
-// PERSIST_: the configuration of the persistent batched kernel (its LDS
-// buffering may differ, below).
+// PERSIST_: the configuration of the persistent batched kernel (its wave
+// placement and LDS buffering may differ, below).
 template <int PREC_, int TA_, int TB_, int BM_, int BN_, bool PERSIST_ = false>
 struct TileCfg {
     static constexpr int PREC = PREC_, TA = TA_, TB = TB_, BM = BM_, BN = BN_;
@@ -339,22 +339,23 @@ struct TileCfg {
     static constexpr int NW = NT / 64;
     static constexpr int GN = BM == BN ? (NW >= 2 ? 2 : 1) : (BM > BN ? 1 : NW), GM = NW / GN;
     static constexpr int WM = BM / 16 / GM, WN = BN / 16 / GN;
-    // Which sub-tiles a wave owns (INTERLEAVE, every mode): wave (g, h) of
-    // the grid owns the 16-row blocks g, g + GM, ... and the 16-column blocks
-    // h, h + GN, ...; a GEMM smaller than the tile covers its top-left
-    // corner, and interleaved blocks spread that corner over the waves, whose
-    // MFMAs skip the sub-tiles outside the GEMM.  (The alternative, kept
-    // behind the flag, is contiguous blocks, rows g*WM .. g*WM + WM - 1, with
-    // every sub-tile of a wave holding part of the GEMM multiplied; on the
-    // 50k U{1..128} batch the interleaved placement took the 128 x 128
-    // launches from 0.593 to 0.556 ms in BF16X6 and 0.529 to 0.521 ms in
-    // BF16X3.)  In an eight-wave workgroup waves w and w + 4 run on one SIMD
+    // Which sub-tiles a wave owns.  FP32 / FP64, and the split modes'
+    // persistent batched kernels (INTERLEAVE): wave (g, h) of the grid owns
+    // the 16-row blocks g, g + GM, ... and the 16-column blocks h, h + GN,
+    // ...; a GEMM smaller than the tile covers its top-left corner, and
+    // interleaved blocks spread that corner over the waves, whose MFMAs skip
+    // the sub-tiles outside the GEMM (on the 50k U{1..128} batch the 128 x
+    // 128 launch went from 0.593 to 0.556 ms in BF16X6 and from 0.529 to
+    // 0.521 ms in BF16X3).  The split modes' single-GEMM and slice kernels,
+    // whose tiles are mostly full, keep contiguous blocks (rows g*WM .. g*WM
+    // + WM - 1) and multiply every sub-tile of a wave holding part of the
+    // GEMM: interleaved, the BF16X3 conv launches ran 7-12% slower.  In an eight-wave workgroup waves w and w + 4 run on one SIMD
     // (the hardware's placement, read from HW_ID on gfx950), so wave w takes
     // grid place g = w / 2, h = (w & 1) ^ (w / 4): each SIMD then holds the
     // blocks (g, h) and (g + 2, 1 - h), and the corner of a GEMM splits
     // evenly over the four SIMDs (on the 50k U{1..128} batch the busiest
     // SIMD's share of the 128 x 128 tiles' MFMAs falls from 42% to 28%).
-    static constexpr bool INTERLEAVE = true;
+    static constexpr bool INTERLEAVE = !SPLIT || PERSIST_;
     __host__ __device__ static constexpr int wave_g(int w)
     {
         return NW == 8 && INTERLEAVE ? w >> 1 : w / GN;

@@ -61,10 +61,10 @@ inline long cdiv(long a, long b) { return (a + b - 1) / b; }
 
 // MFMA cycles one K chunk of a tile costs on its busiest SIMD (waves w and
 // w + 4 of an eight-wave tile share SIMD w % 4), for a GEMM covering mt x nt
-// 16 x 16 blocks of it.  Waves multiply the sub-tiles that reach into the
-// GEMM (with contiguous placement, every sub-tile of a wave holding part of
-// it).  The wave placement is the kernels' own (TileCfg::wave_g, rows_in,
-// ...).
+// 16 x 16 blocks of it.  FP32 / FP64 waves multiply the sub-tiles that reach
+// into the GEMM; the split modes' tile kernels every sub-tile of a wave
+// holding part of it.  The wave placement is the kernels' own
+// (TileCfg::wave_g, rows_in, ...).
 template <class C>
 inline double busiest_simd(long M, long N)
 {

@@ -244,8 +244,10 @@ def test_planner_fills_the_chip():
 # The wave grid of each tile (TileCfg in gemm_core.hpp): 128 x 128 tiles run
 # eight waves in a 4 x 2 grid; the other square tiles four waves in a 2 x 2
 # grid; the tall tiles (128 x 32, 128 x 64) split their rows over four waves,
-# the wide one (32 x 128) its columns.  A wave's sub-tile rows and columns are
-# interleaved with the other waves' (row blocks g, g + GM, ...).
+# the wide one (32 x 128) its columns.  In the split modes' tile kernels a
+# wave owns a contiguous block of 16 x 16 sub-tiles; in FP32 / FP64 its
+# sub-tile rows and columns are interleaved with the other waves' (row blocks
+# g, g + GM, ...).
 WAVE_GRID = {(32, 32): (2, 2), (64, 64): (2, 2), (128, 128): (4, 2), (128, 32): (4, 1),
              (128, 64): (4, 1), (32, 128): (1, 4)}
 
@@ -255,7 +257,11 @@ def idle_fraction(tile, M, N, prec):
     bm, bn = tile
     gm, gn = WAVE_GRID[tile]
     mt, nt = -(-min(bm, M) // 16), -(-min(bn, N) // 16)
-    live_r, live_c = min(gm, mt), min(gn, nt)
+    if prec in (ls.FP32, ls.FP64):
+        live_r, live_c = min(gm, mt), min(gn, nt)
+    else:
+        wm, wn = bm // 16 // gm, bn // 16 // gn
+        live_r, live_c = min(gm, -(-mt // wm)), min(gn, -(-nt // wn))
     return 1.0 - live_r * live_c / (gm * gn)
 
 
