@@ -321,10 +321,12 @@ def test_precision_modes(dev, prec, ta, tb):
         assert bad == 0, ((M, N, K), st)
 
 
-@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3])
-@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 1)])
+@pytest.mark.parametrize("prec", [ls.FP32, ls.BF16X3, ls.BF16X6, ls.BF16])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
 def test_batched_uneven(dev, prec, ta, tb):
-    """Uneven batch, M, N, K ~ U{1..128}; replayed twice: identical results."""
+    """Uneven batch, M, N, K ~ U{1..128}; replayed twice: identical results.
+    Every tile bucket of the persistent kernels (their split modes with the
+    interleaved placement and, in BF16X6's 128 x 128 bucket, one LDS buffer)."""
     rng = np.random.default_rng(17 + prec)
     shapes = [tuple(int(x) for x in rng.integers(1, 129, size=3)) for _ in range(400)]
     outs, Cref, S, Ke, offC = _run_batched(dev, shapes, ta, tb, prec, 555, replays=2)
