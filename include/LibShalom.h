@@ -79,7 +79,9 @@ void LibShalom_sgemm(int transa, int transb, float *C, float *A, float *B, long 
  * the irregular-shape path (tall-skinny / short-wide); the OpenMP Tm x Tn
  * thread grid becomes a workgroup schedule with split-K, or, for a short
  * side of at most 64 and a long K, workgroups of 16 long-side indices that
- * split K between their waves (one launch, no scratch). */
+ * split K between their waves (one launch, no scratch).  LibShalom_sgemm
+ * takes the same plans: on the GPU every call spreads over the whole
+ * device, so the reference's one-core / thread-grid distinction falls away. */
 void LibShalom_sgemm_mp(int transa, int transb, float *C, float *A, float *B, long M, long N, long K);
 
 /* Replaces LibShalom_dgemm (NN_LIB/LibShalom.h:39, NN_SMM.c:30-38). */
